@@ -1,0 +1,847 @@
+// engine.cpp — host side of the MI355X MLS-MPM engine behind the C-ABI of include/mpm_engine.h.
+//
+// Reference counterparts (libs/cuda-engine):
+//   MPMArena slab + carve (cuda-engine.cpp:100-117, 234-263) -> Engine::allocate (one hipMalloc, 256-B carve)
+//   GraphEnv (cuda-engine.h:97-109, cuda-engine.cpp:121-132)  -> Engine stream / events / graph execs
+//   instantiateMPMGraph (MPM.cu:326-374)                      -> Engine::build_graphs (stream capture)
+//   OutputBuffer roll/readTarget/writeTarget (cuda-engine.cpp:160-186) -> write_slot / read_slot below
+//   MPMTickStaging (MPM.cu:19-44)                             -> pinned OutView per slot + hipMemcpyToSymbolAsync
+//   hosePipe (MPM.cu:439-473)                                 -> mpm_hose_pipe
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/mpm_engine.h"
+#include "../../include/mpm_diag.h"
+#include "kernels.h"
+
+using namespace mpm;
+
+struct MpmEngine {
+    MpmParams params{};
+    DevParams dp{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t t0 = nullptr, t1 = nullptr, done = nullptr;
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    int flips = 0;  // store swaps per tick (parity)
+    int cur = 0;    // store holding the current state
+    uint8_t* slab = nullptr;
+    size_t slab_bytes = 0;
+    Store store[2]{};
+    Grid grid{};
+    SortScratch ss{};
+    int32_t* count = nullptr;  // device [2]: slot 0 authoritative, slot 1 staged (cuda-engine.h:287-289)
+    float* out_pos[3]{};
+    float* out_vel[3]{};
+    int32_t* out_num[3]{};
+    OutView* staged = nullptr;  // pinned [3]
+    std::atomic<int32_t> read_slot{-1};
+    int32_t write_slot = 0;
+    int32_t pending_slot = -1;
+    int32_t last_count = 0;
+    int shape = 2;
+    bool has_ticked = false;
+    int32_t last_error = 0;
+    std::string last_msg;
+};
+
+namespace {
+
+thread_local int32_t g_create_error = 0;
+thread_local std::string g_create_msg;
+
+int32_t fail(MpmEngine* h, int32_t code, int32_t detail, const std::string& msg)
+{
+    if (h) {
+        h->last_error = detail ? detail : code;
+        h->last_msg = msg;
+    } else {
+        g_create_error = detail ? detail : code;
+        g_create_msg = msg;
+    }
+    return code;
+}
+
+#define HIP_TRY(h, call)                                                                                         \
+    do {                                                                                                         \
+        hipError_t e_ = (call);                                                                                  \
+        if (e_ != hipSuccess) {                                                                                  \
+            return fail((h), MPM_ERR_DEVICE, (int32_t)e_,                                                        \
+                        std::string(#call) + ": " + hipGetErrorName(e_) + " (" + hipGetErrorString(e_) + ")");  \
+        }                                                                                                        \
+    } while (0)
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) & ~(a - 1); }
+
+struct Carver {
+    uint8_t* base;
+    size_t off = 0;
+    template <class T>
+    T* take(size_t count)
+    {
+        T* r = reinterpret_cast<T*>(base ? base + off : nullptr);
+        off += align_up(count * sizeof(T), 256);
+        return r;
+    }
+};
+
+int ilog2_ceil(uint64_t v)
+{
+    int b = 0;
+    while ((1ull << b) < v) { ++b; }
+    return b;
+}
+
+void derive(const MpmParams& p, DevParams& d)
+{
+    d.W = p.grid_w;
+    d.H = p.grid_h;
+    d.halo = p.halo;
+    d.wall = p.wall;
+    d.G = p.grid_w * p.grid_h;
+    d.num_scenes = p.num_scenes;
+    d.bin_x = p.block_x;
+    d.bin_y = p.block_y;
+    d.nbx = (p.grid_w + p.block_x - 1) / p.block_x;
+    d.nby = (p.grid_h + p.block_y - 1) / p.block_y;
+    d.bps = d.nbx * d.nby;
+    d.dx = p.dx;
+    d.inv_dx = p.inv_dx;
+    d.sim_w = p.sim_w;
+    d.sim_h = p.sim_h;
+    d.dt = p.dt;
+    d.gdt_x = p.gravity_x * p.dt;
+    d.gdt_y = p.gravity_y * p.dt;
+    d.mass = p.mass;
+    d.vol = p.vol;
+    d.tait_k = p.tait_k;
+    d.tait_gamma = p.tait_gamma;
+    d.p_min = p.p_min;
+    d.p_max = p.p_max;
+    d.mu = p.mu;
+    d.lambda = p.lambda;
+    d.theta_c = p.theta_c;
+    d.theta_s = p.theta_s;
+    d.hardening = p.hardening;
+    d.j_min = p.j_min;
+    d.j_max = p.j_max;
+    d.closure = p.closure;
+}
+
+// Validates p (after defaults are filled); returns an error message or empty.
+std::string validate(MpmParams& p)
+{
+    if (p.block_x == 0 || p.block_y == 0) {
+        p.block_x = 16;
+        p.block_y = 16;
+    }
+    if (p.chunk == 0) { p.chunk = 512; }
+    if (p.sort_interval <= 0) { p.sort_interval = p.substeps; }
+    if (p.out_chunk <= 0) { p.out_chunk = 1536; }
+    if (p.grid_w < 8 || p.grid_h < 8) { return "grid_w and grid_h must be >= 8"; }
+    if ((int64_t)p.grid_w * p.grid_h > (1ll << 30)) { return "grid too large"; }
+    if (p.halo < 2) { return "halo must be >= 2 (quadratic stencil reach)"; }
+    if (p.wall < 0 || 2 * p.wall + 2 >= p.grid_w || 2 * p.wall + 2 >= p.grid_h) { return "wall out of range"; }
+    if (!(p.dx > 0.f) || !(p.inv_dx > 0.f) || !(p.dt > 0.f)) { return "dx, inv_dx and dt must be > 0"; }
+    if (!(p.sim_w > 0.f) || !(p.sim_h > 0.f)) { return "sim_w and sim_h must be > 0"; }
+    if ((double)p.sim_w * p.inv_dx + p.halo + 1.5 > (double)p.grid_w ||
+        (double)p.sim_h * p.inv_dx + p.halo + 1.5 > (double)p.grid_h) {
+        return "domain does not fit the grid: need sim * inv_dx + halo + 1.5 <= grid";
+    }
+    if (!(p.mass > 0.f) || !(p.vol > 0.f)) { return "mass and vol must be > 0"; }
+    if (!(p.j_min > 0.f) || !(p.j_max >= p.j_min)) { return "J clamp range invalid"; }
+    if (p.substeps < 1 || p.substeps > 100000) { return "substeps out of range"; }
+    if (p.capacity < 1 || p.capacity > 1000000000) { return "capacity out of range"; }
+    if (p.num_scenes < 1 || p.num_scenes > kMaxScenes) { return "num_scenes out of range"; }
+    if (p.mode < MPM_MODE_REFERENCE || p.mode > MPM_MODE_FUSED) { return "unknown mode"; }
+    if (p.closure != MPM_CLOSURE_ANALYTIC && p.closure != MPM_CLOSURE_LEARNED) { return "unknown closure"; }
+    if (shape_index(p.block_x, p.block_y) < 0) { return "block_x x block_y is not a compiled tile shape"; }
+    if (p.chunk < 64 || p.chunk > 65536) { return "chunk out of range"; }
+    const int64_t K = (int64_t)p.num_scenes * ((p.grid_w + p.block_x - 1) / p.block_x) *
+                      ((p.grid_h + p.block_y - 1) / p.block_y);
+    if (K > (1ll << 30)) { return "too many bins"; }
+    return "";
+}
+
+int32_t allocate(MpmEngine* h)
+{
+    const MpmParams& p = h->params;
+    const size_t cap = (size_t)p.capacity;
+    const size_t nodes = (size_t)h->dp.G * (size_t)p.num_scenes;
+    const int K = h->dp.bps * p.num_scenes;
+    SortScratch& ss = h->ss;
+    ss.capacity = p.capacity;
+    ss.ntiles = (int)((cap + kRadixTile - 1) / kRadixTile);
+    ss.num_keys = K;
+    ss.key_bits = std::max(1, ilog2_ceil((uint64_t)K));
+    ss.chunk = p.chunk;
+    ss.max_items = (int)((cap + p.chunk - 1) / p.chunk) + K;
+    const size_t scan_len = std::max((size_t)kRadixBins * ss.ntiles, (size_t)K);
+    const size_t n_partials = scan_len / (kThreads * 16) + 2;
+    const size_t out_chunks = (cap + p.out_chunk - 1) / p.out_chunk;
+
+    for (int pass = 0; pass < 2; ++pass) {
+        Carver c{pass == 0 ? nullptr : h->slab};
+        for (int s = 0; s < 2; ++s) {
+            Store& st = h->store[s];
+            float** planes[13] = {&st.x, &st.y, &st.vx, &st.vy, &st.c00, &st.c01, &st.c10, &st.c11,
+                                  &st.f00, &st.f01, &st.f10, &st.f11, &st.J};
+            for (float** pl : planes) { *pl = c.take<float>(cap); }
+            st.mat = c.take<uint8_t>(cap);
+            st.id = c.take<int32_t>(cap);
+            st.key = c.take<uint32_t>(cap);
+        }
+        h->grid.mom_x = c.take<float>(nodes);
+        h->grid.mom_y = c.take<float>(nodes);
+        h->grid.mass = c.take<float>(nodes);
+        h->grid.vel = c.take<float2>(nodes);
+        h->grid.vmass = c.take<float>(nodes);
+        ss.keys_tmp = c.take<uint32_t>(cap);
+        ss.keys_alt = c.take<uint32_t>(cap);
+        ss.vals_tmp = c.take<int32_t>(cap);
+        ss.vals_alt = c.take<int32_t>(cap);
+        ss.perm = c.take<int32_t>(cap);
+        ss.hist = c.take<int32_t>((size_t)kRadixBins * ss.ntiles);
+        ss.partials = c.take<int32_t>(n_partials);
+        ss.bin_start = c.take<int32_t>((size_t)K);
+        ss.bin_end = c.take<int32_t>((size_t)K);
+        ss.item_cnt = c.take<int32_t>((size_t)K);
+        ss.items = c.take<Item>((size_t)ss.max_items);
+        ss.num_items = c.take<int32_t>(1);
+        h->count = c.take<int32_t>(2);
+        for (int k = 0; k < 3; ++k) {
+            h->out_pos[k] = c.take<float>(2 * cap);
+            h->out_vel[k] = c.take<float>(2 * cap);
+            h->out_num[k] = c.take<int32_t>(out_chunks);
+        }
+        if (pass == 0) {
+            h->slab_bytes = c.off;
+            HIP_TRY(h, hipMalloc((void**)&h->slab, h->slab_bytes));
+        }
+    }
+    // Zero the whole slab once (grid accumulators must start at zero; counts at zero).
+    HIP_TRY(h, hipMemsetAsync(h->slab, 0, h->slab_bytes, h->stream));
+    HIP_TRY(h, hipHostMalloc((void**)&h->staged, 3 * sizeof(OutView), hipHostMallocDefault));
+    for (int k = 0; k < 3; ++k) {
+        h->staged[k].pos = h->out_pos[k];
+        h->staged[k].vel = h->out_vel[k];
+        h->staged[k].num_atoms = h->out_num[k];
+        h->staged[k].out_chunk = p.out_chunk;
+        h->staged[k].pad = 0;
+    }
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    return MPM_OK;
+}
+
+// Optional per-launch HIP event pairs (mpm_time_kernels); absent during graph capture.
+struct LaunchTimer {
+    hipStream_t st = nullptr;
+    std::vector<hipEvent_t> ev;
+    std::vector<int> cls;
+    hipError_t begin(int k)
+    {
+        hipEvent_t a, b;
+        hipError_t e = hipEventCreate(&a);
+        if (e != hipSuccess) { return e; }
+        e = hipEventCreate(&b);
+        if (e != hipSuccess) { return e; }
+        ev.push_back(a);
+        ev.push_back(b);
+        cls.push_back(k);
+        return hipEventRecord(a, st);
+    }
+    hipError_t end() { return hipEventRecord(ev.back(), st); }
+    ~LaunchTimer()
+    {
+        for (hipEvent_t e : ev) { (void)hipEventDestroy(e); }
+    }
+};
+
+// Enqueue `substeps` substeps on the stream starting from store c (c is updated to the store holding the
+// result), then the output copy when `output` is set.
+hipError_t enqueue_substeps(MpmEngine* h, int& c, int substeps, bool output, LaunchTimer* tm)
+{
+    const MpmParams& p = h->params;
+    const DevParams& d = h->dp;
+    hipStream_t st = h->stream;
+    const int cap = p.capacity;
+    hipError_t e = hipSuccess;
+#define ENQ(k, x)                                        \
+    do {                                                 \
+        if (tm && (e = tm->begin(k)) != hipSuccess) { return e; } \
+        e = (x);                                         \
+        if (e != hipSuccess) { return e; }               \
+        if (tm && (e = tm->end()) != hipSuccess) { return e; }    \
+    } while (0)
+    if (p.mode == MPM_MODE_REFERENCE) {
+        // Reference graph (MPM.cu:357-369): P2G -> grid -> G2P per substep; the reset is folded into the
+        // grid update, which zeroes the accumulators after reading them.
+        for (int s = 0; s < substeps; ++s) {
+            ENQ(MPM_KT_P2G, launch_p2g_reference(st, d, h->store[c], h->count, cap, h->grid));
+            ENQ(MPM_KT_GRID, launch_grid_update(st, d, h->grid));
+            ENQ(MPM_KT_G2P, launch_g2p_reference(st, d, h->store[c], h->count, cap, h->grid));
+        }
+    } else {
+        int done = 0;
+        while (done < substeps) {
+            const int L = std::min(p.sort_interval, substeps - done);
+            ENQ(MPM_KT_BINNING, launch_binning(st, d, h->store[c], h->store[c ^ 1], h->count, h->ss));
+            c ^= 1;
+            const Store& s = h->store[c];
+            if (p.mode == MPM_MODE_TILED) {
+                for (int k = 0; k < L; ++k) {
+                    ENQ(MPM_KT_P2G, launch_p2g_tiled(st, h->shape, d, s, h->grid, h->ss, h->count));
+                    ENQ(MPM_KT_GRID, launch_grid_update(st, d, h->grid));
+                    ENQ(MPM_KT_G2P, launch_g2p_tiled(st, h->shape, d, s, h->grid, h->ss, h->count));
+                }
+            } else {
+                ENQ(MPM_KT_P2G, launch_p2g_tiled(st, h->shape, d, s, h->grid, h->ss, h->count));
+                for (int k = 0; k < L - 1; ++k) {
+                    ENQ(MPM_KT_GRID, launch_grid_update(st, d, h->grid));
+                    ENQ(MPM_KT_G2P2G, launch_g2p2g(st, h->shape, d, s, h->grid, h->ss, h->count));
+                }
+                ENQ(MPM_KT_GRID, launch_grid_update(st, d, h->grid));
+                ENQ(MPM_KT_G2P, launch_g2p_tiled(st, h->shape, d, s, h->grid, h->ss, h->count));
+            }
+            done += L;
+        }
+    }
+    if (output) { ENQ(MPM_KT_OUTPUT, launch_output(st, h->store[c], h->count, cap)); }
+#undef ENQ
+    return hipSuccess;
+}
+
+int32_t build_graphs(MpmEngine* h)
+{
+    const MpmParams& p = h->params;
+    int segs = 0;
+    if (p.mode != MPM_MODE_REFERENCE) { segs = (p.substeps + p.sort_interval - 1) / p.sort_interval; }
+    h->flips = segs & 1;
+    for (int c = 0; c < 2; ++c) {
+        HIP_TRY(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        int cc = c;
+        hipError_t e = enqueue_substeps(h, cc, p.substeps, true, nullptr);
+        hipGraph_t g = nullptr;
+        hipError_t e2 = hipStreamEndCapture(h->stream, &g);
+        if (e != hipSuccess) {
+            if (g) { (void)hipGraphDestroy(g); }
+            return fail(h, MPM_ERR_DEVICE, (int32_t)e, std::string("tick capture: ") + hipGetErrorName(e));
+        }
+        if (e2 != hipSuccess) { return fail(h, MPM_ERR_DEVICE, (int32_t)e2, "hipStreamEndCapture failed"); }
+        h->graph[c] = g;
+        HIP_TRY(h, hipGraphInstantiate(&h->exec[c], g, nullptr, nullptr, 0));
+    }
+    return MPM_OK;
+}
+
+void release(MpmEngine* h)
+{
+    if (!h) { return; }
+    if (h->stream) { (void)hipStreamSynchronize(h->stream); }
+    for (int c = 0; c < 2; ++c) {
+        if (h->exec[c]) { (void)hipGraphExecDestroy(h->exec[c]); }
+        if (h->graph[c]) { (void)hipGraphDestroy(h->graph[c]); }
+    }
+    if (h->t0) { (void)hipEventDestroy(h->t0); }
+    if (h->t1) { (void)hipEventDestroy(h->t1); }
+    if (h->done) { (void)hipEventDestroy(h->done); }
+    if (h->slab) { (void)hipFree(h->slab); }
+    if (h->staged) { (void)hipHostFree(h->staged); }
+    if (h->stream) { (void)hipStreamDestroy(h->stream); }
+}
+
+float f32_ref_dt() { return 0.032f / 20.f; }
+
+}  // namespace
+
+extern "C" {
+
+int32_t mpm_params_preset(int32_t preset, MpmParams* out)
+{
+    if (!out) { return MPM_ERR_INVALID_ARG; }
+    MpmParams p;
+    std::memset(&p, 0, sizeof(p));
+    // Fields common to the unit-square presets.
+    auto unit_square = [&](int cells, float dt, float rho) {
+        p.grid_w = p.grid_h = cells + 4;
+        p.halo = 2;
+        p.wall = 2;
+        p.dx = 1.f / (float)cells;
+        p.inv_dx = (float)cells;
+        p.sim_w = p.sim_h = 1.f;
+        p.dt = dt;
+        p.gravity_x = 0.f;
+        p.gravity_y = -9.8f;
+        p.vol = p.dx * p.dx / 4.f;
+        p.mass = rho * p.vol;
+        // fluid: bulk modulus 400 as Tait with gamma = 1 (tau = k (J - 1) I), no tension (as the reference)
+        p.tait_k = 400.f;
+        p.tait_gamma = 1.f;
+        p.p_min = -0.f;
+        p.p_max = 4000.f;
+        p.theta_c = 2.5e-2f;
+        p.theta_s = 7.5e-3f;
+        p.hardening = 10.f;
+        p.j_min = 0.1f;
+        p.j_max = 10.f;
+        p.num_scenes = 1;
+        p.mode = MPM_MODE_FUSED;
+        p.closure = MPM_CLOSURE_ANALYTIC;
+        p.block_x = 16;
+        p.block_y = 16;
+        p.chunk = 512;
+        p.out_chunk = 1536;
+    };
+    auto lame = [&](float E, float nu) {
+        p.mu = E / (2.f * (1.f + nu));
+        p.lambda = E * nu / ((1.f + nu) * (1.f - 2.f * nu));
+    };
+    switch (preset) {
+    case MPM_PRESET_REFERENCE:
+        // constants.h:9-88 and MPM.cu:336-337 (sim 384 x 192, grid 196 x 100, dx 2, halo 2)
+        p.grid_w = 384 / 2 + 4;
+        p.grid_h = 192 / 2 + 4;
+        p.halo = 2;
+        p.wall = 2;
+        p.dx = 2.f;
+        p.inv_dx = 1.f / 2.f;
+        p.sim_w = 384.f;
+        p.sim_h = 192.f;
+        p.dt = f32_ref_dt();
+        p.gravity_x = 0.f;
+        p.gravity_y = 700.f;
+        p.mass = 1.f;
+        p.vol = 1.2f;
+        p.tait_k = 8.2e4f;
+        p.tait_gamma = 3.2f;
+        p.p_min = -0.f * 8.2e4f;
+        p.p_max = 10.f * 8.2e4f;
+        p.j_min = 0.1f;
+        p.j_max = 10.f;
+        p.substeps = 20;
+        p.capacity = 400 * 1000;
+        p.num_scenes = 1;
+        p.sort_interval = 20;
+        p.mode = MPM_MODE_FUSED;
+        p.closure = MPM_CLOSURE_ANALYTIC;
+        p.block_x = 16;
+        p.block_y = 16;
+        p.chunk = 512;
+        p.out_chunk = 128 * 12;
+        break;
+    case MPM_PRESET_JELLY128:
+        unit_square(128, 1e-4f, 1.f);
+        lame(1e3f, 0.2f);
+        p.substeps = 100;
+        p.capacity = 32768;
+        p.sort_interval = 50;
+        break;
+    case MPM_PRESET_MIXED1024:
+        unit_square(1024, 2e-5f, 4.f);
+        lame(1e4f, 0.2f);
+        p.substeps = 100;
+        p.capacity = 4194304;
+        p.sort_interval = 25;
+        break;
+    case MPM_PRESET_BATCH64X256:
+        unit_square(256, 2e-5f, 4.f);
+        lame(1e4f, 0.2f);
+        p.substeps = 200;
+        p.num_scenes = 64;
+        p.capacity = 64 * 524288;
+        p.sort_interval = 25;
+        break;
+    case MPM_PRESET_LEARNED1024:
+        unit_square(1024, 2e-5f, 4.f);
+        lame(1e4f, 0.2f);
+        p.substeps = 100;
+        p.capacity = 4194304;
+        p.sort_interval = 25;
+        p.closure = MPM_CLOSURE_LEARNED;
+        break;
+    case MPM_PRESET_LEARNED16X2048:
+        unit_square(2048, 1e-5f, 4.f);
+        lame(1e4f, 0.2f);
+        p.substeps = 100;
+        p.num_scenes = 16;
+        p.capacity = 16 * 16777216 / 2;
+        p.sort_interval = 25;
+        p.closure = MPM_CLOSURE_LEARNED;
+        break;
+    default:
+        return MPM_ERR_INVALID_ARG;
+    }
+    *out = p;
+    return MPM_OK;
+}
+
+int32_t mpm_create(const MpmParams* params, MpmEngine** out)
+{
+    if (!params || !out) { return fail(nullptr, MPM_ERR_INVALID_ARG, 0, "null argument"); }
+    *out = nullptr;
+    MpmParams p = *params;
+    const std::string bad = validate(p);
+    if (!bad.empty()) { return fail(nullptr, MPM_ERR_INVALID_ARG, 0, bad); }
+    if (p.closure == MPM_CLOSURE_LEARNED) {
+        return fail(nullptr, MPM_ERR_NOT_BUILT, 0, "learned closure is not compiled into this library yet");
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        return fail(nullptr, MPM_ERR_NO_DEVICE, 0, "no HIP device");
+    }
+    MpmEngine* h = new (std::nothrow) MpmEngine();
+    if (!h) { return fail(nullptr, MPM_ERR_INVALID_ARG, 0, "out of host memory"); }
+    h->params = p;
+    derive(p, h->dp);
+    h->shape = shape_index(p.block_x, p.block_y);
+    int32_t rc = MPM_OK;
+    auto bail = [&](int32_t code) {
+        g_create_error = h->last_error;
+        g_create_msg = h->last_msg;
+        release(h);
+        delete h;
+        return code;
+    };
+    if (hipGetDevice(&h->device) != hipSuccess) { return bail(fail(h, MPM_ERR_DEVICE, 0, "hipGetDevice")); }
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+        return bail(fail(h, MPM_ERR_DEVICE, 0, "hipStreamCreateWithFlags"));
+    }
+    if (hipEventCreate(&h->t0) != hipSuccess || hipEventCreate(&h->t1) != hipSuccess ||
+        hipEventCreateWithFlags(&h->done, hipEventDisableTiming) != hipSuccess) {
+        return bail(fail(h, MPM_ERR_DEVICE, 0, "hipEventCreate"));
+    }
+    if ((rc = allocate(h)) != MPM_OK) { return bail(rc); }
+    if ((rc = build_graphs(h)) != MPM_OK) { return bail(rc); }
+    *out = h;
+    return MPM_OK;
+}
+
+int32_t mpm_destroy(MpmEngine* h)
+{
+    if (!h) { return MPM_ERR_INVALID_ARG; }
+    release(h);
+    delete h;
+    return MPM_OK;
+}
+
+int32_t mpm_set_particles(MpmEngine* h, int32_t n, const float* x, const float* v, const float* C, const float* F,
+                          const float* J, const uint8_t* material, const int32_t* scene_counts)
+{
+    if (!h) { return MPM_ERR_INVALID_ARG; }
+    const MpmParams& p = h->params;
+    if (n < 0 || (n > 0 && !x)) { return fail(h, MPM_ERR_INVALID_ARG, 0, "n < 0 or x is NULL"); }
+    if (n > p.capacity) { return fail(h, MPM_ERR_CAPACITY, 0, "n exceeds capacity"); }
+    if (p.num_scenes > 1 && !scene_counts) { return fail(h, MPM_ERR_INVALID_ARG, 0, "scene_counts required"); }
+    std::vector<uint32_t> scene_key((size_t)n, 0u);
+    if (scene_counts) {
+        int64_t total = 0;
+        for (int s = 0; s < p.num_scenes; ++s) {
+            if (scene_counts[s] < 0) { return fail(h, MPM_ERR_INVALID_ARG, 0, "negative scene count"); }
+            for (int32_t k = 0; k < scene_counts[s]; ++k) {
+                if (total + k < n) { scene_key[(size_t)(total + k)] = (uint32_t)s * (uint32_t)h->dp.bps; }
+            }
+            total += scene_counts[s];
+        }
+        if (total != n) { return fail(h, MPM_ERR_INVALID_ARG, 0, "scene_counts do not sum to n"); }
+    }
+    for (int32_t i = 0; i < n; ++i) {
+        const float px = x[2 * i], py = x[2 * i + 1];
+        if (!(px >= 0.f && px <= p.sim_w && py >= 0.f && py <= p.sim_h)) {
+            return fail(h, MPM_ERR_INVALID_ARG, 0, "particle position outside [0,sim_w]x[0,sim_h] or not finite");
+        }
+        if (material && material[i] > MPM_MAT_SNOW) { return fail(h, MPM_ERR_INVALID_ARG, 0, "unknown material"); }
+    }
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    const size_t N = (size_t)n;
+    std::vector<float> planes(13 * N);
+    for (size_t i = 0; i < N; ++i) {
+        planes[0 * N + i] = x[2 * i];
+        planes[1 * N + i] = x[2 * i + 1];
+        planes[2 * N + i] = v ? v[2 * i] : 0.f;
+        planes[3 * N + i] = v ? v[2 * i + 1] : 0.f;
+        for (int k = 0; k < 4; ++k) {
+            planes[(4 + k) * N + i] = C ? C[4 * i + k] : 0.f;
+            planes[(8 + k) * N + i] = F ? F[4 * i + k] : ((k == 0 || k == 3) ? 1.f : 0.f);
+        }
+        planes[12 * N + i] = J ? J[i] : 1.f;
+    }
+    std::vector<uint8_t> mat(N);
+    std::vector<int32_t> ids(N);
+    for (size_t i = 0; i < N; ++i) {
+        mat[i] = material ? material[i] : (uint8_t)MPM_MAT_FLUID;
+        ids[i] = (int32_t)i;
+    }
+    Store& s = h->store[h->cur];
+    float* dst[13] = {s.x, s.y, s.vx, s.vy, s.c00, s.c01, s.c10, s.c11, s.f00, s.f01, s.f10, s.f11, s.J};
+    if (n > 0) {
+        for (int k = 0; k < 13; ++k) {
+            HIP_TRY(h, hipMemcpy(dst[k], planes.data() + (size_t)k * N, N * sizeof(float), hipMemcpyHostToDevice));
+        }
+        HIP_TRY(h, hipMemcpy(s.mat, mat.data(), N, hipMemcpyHostToDevice));
+        HIP_TRY(h, hipMemcpy(s.id, ids.data(), N * sizeof(int32_t), hipMemcpyHostToDevice));
+        HIP_TRY(h, hipMemcpy(s.key, scene_key.data(), N * sizeof(uint32_t), hipMemcpyHostToDevice));
+    }
+    const int32_t cnt[2] = {n, n};
+    HIP_TRY(h, hipMemcpy(h->count, cnt, sizeof(cnt), hipMemcpyHostToDevice));
+    h->last_count = n;
+    return MPM_OK;
+}
+
+int32_t mpm_step(MpmEngine* h, int32_t n_ticks)
+{
+    if (!h) { return MPM_ERR_INVALID_ARG; }
+    if (n_ticks < 0) { return fail(h, MPM_ERR_INVALID_ARG, 0, "n_ticks < 0"); }
+    for (int32_t k = 0; k < n_ticks; ++k) {
+        const int32_t slot = h->write_slot;
+        HIP_TRY(h, set_output_view_async(h->stream, &h->staged[slot]));
+        const bool last = k == n_ticks - 1;
+        if (last) { HIP_TRY(h, hipEventRecord(h->t0, h->stream)); }
+        HIP_TRY(h, hipGraphLaunch(h->exec[h->cur], h->stream));
+        if (last) { HIP_TRY(h, hipEventRecord(h->t1, h->stream)); }
+        h->cur ^= h->flips;
+        h->pending_slot = slot;
+        h->write_slot = (slot + 1) % 3;
+        h->has_ticked = true;
+    }
+    HIP_TRY(h, hipEventRecord(h->done, h->stream));
+    return MPM_OK;
+}
+
+int32_t mpm_query(MpmEngine* h)
+{
+    if (!h) { return MPM_ERR_INVALID_ARG; }
+    const hipError_t e = hipEventQuery(h->done);
+    if (e == hipErrorNotReady) { return MPM_BUSY; }
+    if (e != hipSuccess) { return fail(h, MPM_ERR_DEVICE, (int32_t)e, "hipEventQuery"); }
+    if (h->pending_slot >= 0) { h->read_slot.store(h->pending_slot, std::memory_order_release); }
+    return MPM_OK;
+}
+
+int32_t mpm_sync(MpmEngine* h)
+{
+    if (!h) { return MPM_ERR_INVALID_ARG; }
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    if (h->pending_slot >= 0) { h->read_slot.store(h->pending_slot, std::memory_order_release); }
+    return MPM_OK;
+}
+
+int32_t mpm_last_step_ms(MpmEngine* h, float* ms)
+{
+    if (!h || !ms) { return MPM_ERR_INVALID_ARG; }
+    if (!h->has_ticked) { return fail(h, MPM_ERR_STATE, 0, "no tick has run"); }
+    HIP_TRY(h, hipEventSynchronize(h->t1));
+    HIP_TRY(h, hipEventElapsedTime(ms, h->t0, h->t1));
+    return MPM_OK;
+}
+
+int32_t mpm_read_view(MpmEngine* h, MpmView* out)
+{
+    if (!h || !out) { return MPM_ERR_INVALID_ARG; }
+    const int32_t s = h->read_slot.load(std::memory_order_acquire);
+    out->slot = s;
+    if (s < 0) {
+        out->pos = nullptr;
+        out->vel = nullptr;
+        out->num_atoms = nullptr;
+        out->count = 0;
+        return MPM_OK;
+    }
+    out->pos = h->out_pos[s];
+    out->vel = h->out_vel[s];
+    out->num_atoms = h->out_num[s];
+    out->count = h->last_count;
+    return MPM_OK;
+}
+
+int32_t mpm_count(MpmEngine* h, int32_t* n)
+{
+    if (!h || !n) { return MPM_ERR_INVALID_ARG; }
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    int32_t c[2];
+    HIP_TRY(h, hipMemcpy(c, h->count, sizeof(c), hipMemcpyDeviceToHost));
+    *n = c[0];
+    h->last_count = c[0];
+    return MPM_OK;
+}
+
+int32_t mpm_readback(MpmEngine* h, float* x, float* v, float* C, float* F, float* J, uint8_t* material, int32_t* n)
+{
+    if (!h) { return MPM_ERR_INVALID_ARG; }
+    int32_t cnt = 0;
+    int32_t rc = mpm_count(h, &cnt);
+    if (rc != MPM_OK) { return rc; }
+    if (n) { *n = cnt; }
+    const size_t N = (size_t)cnt;
+    if (N == 0) { return MPM_OK; }
+    const Store& s = h->store[h->cur];
+    std::vector<int32_t> ids(N);
+    HIP_TRY(h, hipMemcpy(ids.data(), s.id, N * sizeof(int32_t), hipMemcpyDeviceToHost));
+    for (size_t j = 0; j < N; ++j) {
+        if (ids[j] < 0 || (size_t)ids[j] >= N) { return fail(h, MPM_ERR_STATE, 0, "particle ids corrupted"); }
+    }
+    std::vector<float> tmp(N);
+    auto fetch = [&](const float* src, float* dst, int stride, int k) -> int32_t {
+        HIP_TRY(h, hipMemcpy(tmp.data(), src, N * sizeof(float), hipMemcpyDeviceToHost));
+        for (size_t j = 0; j < N; ++j) { dst[(size_t)ids[j] * stride + k] = tmp[j]; }
+        return MPM_OK;
+    };
+    if (x && ((rc = fetch(s.x, x, 2, 0)) || (rc = fetch(s.y, x, 2, 1)))) { return rc; }
+    if (v && ((rc = fetch(s.vx, v, 2, 0)) || (rc = fetch(s.vy, v, 2, 1)))) { return rc; }
+    if (C && ((rc = fetch(s.c00, C, 4, 0)) || (rc = fetch(s.c01, C, 4, 1)) || (rc = fetch(s.c10, C, 4, 2)) ||
+              (rc = fetch(s.c11, C, 4, 3)))) {
+        return rc;
+    }
+    if (F && ((rc = fetch(s.f00, F, 4, 0)) || (rc = fetch(s.f01, F, 4, 1)) || (rc = fetch(s.f10, F, 4, 2)) ||
+              (rc = fetch(s.f11, F, 4, 3)))) {
+        return rc;
+    }
+    if (J && (rc = fetch(s.J, J, 1, 0))) { return rc; }
+    if (material) {
+        std::vector<uint8_t> m(N);
+        HIP_TRY(h, hipMemcpy(m.data(), s.mat, N, hipMemcpyDeviceToHost));
+        for (size_t j = 0; j < N; ++j) { material[ids[j]] = m[j]; }
+    }
+    return MPM_OK;
+}
+
+int32_t mpm_read_grid(MpmEngine* h, float* mass, float* vel_x, float* vel_y)
+{
+    if (!h) { return MPM_ERR_INVALID_ARG; }
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    const size_t nodes = (size_t)h->dp.G * (size_t)h->params.num_scenes;
+    if (mass) { HIP_TRY(h, hipMemcpy(mass, h->grid.vmass, nodes * sizeof(float), hipMemcpyDeviceToHost)); }
+    if (vel_x || vel_y) {
+        std::vector<float2> v(nodes);
+        HIP_TRY(h, hipMemcpy(v.data(), h->grid.vel, nodes * sizeof(float2), hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < nodes; ++k) {
+            if (vel_x) { vel_x[k] = v[k].x; }
+            if (vel_y) { vel_y[k] = v[k].y; }
+        }
+    }
+    return MPM_OK;
+}
+
+int32_t mpm_hose_pipe(MpmEngine* h, float origin_x, float origin_y, float velocity_x, float velocity_y,
+                      float nozzle_width, float spacing, float jitter, uint64_t tick, int32_t* emitted)
+{
+    if (!h) { return MPM_ERR_INVALID_ARG; }
+    if (!(spacing > 0.f) || !(nozzle_width >= 0.f) || !(jitter >= 0.f)) {
+        return fail(h, MPM_ERR_INVALID_ARG, 0, "spacing must be > 0, width and jitter >= 0");
+    }
+    // hosePipe, MPM.cu:450-453
+    const float speed = std::sqrt(velocity_x * velocity_x + velocity_y * velocity_y);
+    const int32_t width = std::max(1, (int32_t)std::lround(nozzle_width / spacing));
+    const int32_t rows = std::max(1, (int32_t)std::lround(speed / spacing));
+    const int32_t count = width * rows;
+    const float tick_len = (float)h->params.substeps * h->params.dt;
+    const hipError_t e = launch_emitter(h->stream, h->dp, h->store[h->cur], h->count, h->params.capacity, origin_x,
+                                        origin_y, velocity_x, velocity_y, width, spacing, jitter, count, tick,
+                                        1.f / tick_len);
+    if (e != hipSuccess) { return fail(h, MPM_ERR_DEVICE, (int32_t)e, "emitter launch"); }
+    if (emitted) { *emitted = count; }
+    return MPM_OK;
+}
+
+int32_t mpm_last_error(MpmEngine* h) { return h ? h->last_error : g_create_error; }
+
+const char* mpm_error_name(int32_t err)
+{
+    switch (err) {
+    case MPM_OK: return "MPM_OK";
+    case MPM_BUSY: return "MPM_BUSY";
+    case MPM_ERR_INVALID_ARG: return "MPM_ERR_INVALID_ARG";
+    case MPM_ERR_DEVICE: return "MPM_ERR_DEVICE";
+    case MPM_ERR_CAPACITY: return "MPM_ERR_CAPACITY";
+    case MPM_ERR_NO_DEVICE: return "MPM_ERR_NO_DEVICE";
+    case MPM_ERR_STATE: return "MPM_ERR_STATE";
+    case MPM_ERR_NOT_BUILT: return "MPM_ERR_NOT_BUILT";
+    default: return hipGetErrorName((hipError_t)err);
+    }
+}
+
+const char* mpm_last_message(MpmEngine* h) { return h ? h->last_msg.c_str() : g_create_msg.c_str(); }
+
+int32_t mpm_debug_bin(MpmEngine* h, uint32_t* keys, int32_t* ids, int32_t* n)
+{
+    if (!h) { return MPM_ERR_INVALID_ARG; }
+    if (h->params.mode == MPM_MODE_REFERENCE) { return fail(h, MPM_ERR_STATE, 0, "no binning in reference mode"); }
+    int32_t cnt = 0;
+    int32_t rc = mpm_count(h, &cnt);
+    if (rc != MPM_OK) { return rc; }
+    const int c = h->cur;
+    HIP_TRY(h, launch_binning(h->stream, h->dp, h->store[c], h->store[c ^ 1], h->count, h->ss));
+    h->cur ^= 1;
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    const Store& s = h->store[h->cur];
+    if (keys && cnt) { HIP_TRY(h, hipMemcpy(keys, s.key, (size_t)cnt * sizeof(uint32_t), hipMemcpyDeviceToHost)); }
+    if (ids && cnt) { HIP_TRY(h, hipMemcpy(ids, s.id, (size_t)cnt * sizeof(int32_t), hipMemcpyDeviceToHost)); }
+    if (n) { *n = cnt; }
+    return MPM_OK;
+}
+
+int32_t mpm_time_kernels(MpmEngine* h, int32_t substeps, float* ms, int32_t* launches)
+{
+    if (!h || !ms || !launches || substeps < 1) { return MPM_ERR_INVALID_ARG; }
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    LaunchTimer tm;
+    tm.st = h->stream;
+    int c = h->cur;
+    const hipError_t e = enqueue_substeps(h, c, substeps, false, &tm);
+    if (e != hipSuccess) { return fail(h, MPM_ERR_DEVICE, (int32_t)e, "timed launch"); }
+    h->cur = c;
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    for (int k = 0; k < MPM_KT_COUNT; ++k) {
+        ms[k] = 0.f;
+        launches[k] = 0;
+    }
+    for (size_t i = 0; i < tm.cls.size(); ++i) {
+        float t = 0.f;
+        HIP_TRY(h, hipEventElapsedTime(&t, tm.ev[2 * i], tm.ev[2 * i + 1]));
+        ms[tm.cls[i]] += t;
+        launches[tm.cls[i]] += 1;
+    }
+    return MPM_OK;
+}
+
+int32_t mpm_compute_stress(MpmEngine* h, int32_t n, const uint8_t* material, const float* F, const float* J,
+                           float* tau)
+{
+    if (!h || n < 0 || (n > 0 && (!material || !F || !J || !tau))) { return MPM_ERR_INVALID_ARG; }
+    if (n == 0) { return MPM_OK; }
+    const size_t N = (size_t)n;
+    std::vector<float> planes(5 * N);
+    for (size_t i = 0; i < N; ++i) {
+        for (int k = 0; k < 4; ++k) { planes[k * N + i] = F[4 * i + k]; }
+        planes[4 * N + i] = J[i];
+    }
+    uint8_t* buf = nullptr;
+    const size_t bytes = 8 * N * sizeof(float) + N;
+    HIP_TRY(h, hipMalloc((void**)&buf, bytes));
+    float* dF = reinterpret_cast<float*>(buf);
+    float* dJ = dF + 4 * N;
+    float* dT = dJ + N;
+    uint8_t* dM = reinterpret_cast<uint8_t*>(dT + 3 * N);
+    hipError_t e = hipMemcpy(dF, planes.data(), 5 * N * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) { e = hipMemcpy(dM, material, N, hipMemcpyHostToDevice); }
+    if (e == hipSuccess) { e = launch_stress(h->stream, h->dp, n, dM, dF, dJ, dT); }
+    if (e == hipSuccess) { e = hipStreamSynchronize(h->stream); }
+    std::vector<float> t(3 * N);
+    if (e == hipSuccess) { e = hipMemcpy(t.data(), dT, 3 * N * sizeof(float), hipMemcpyDeviceToHost); }
+    (void)hipFree(buf);
+    if (e != hipSuccess) { return fail(h, MPM_ERR_DEVICE, (int32_t)e, "stress kernel"); }
+    for (size_t i = 0; i < N; ++i) {
+        for (int k = 0; k < 3; ++k) { tau[3 * i + k] = t[k * N + i]; }
+    }
+    return MPM_OK;
+}
+
+}  // extern "C"
+

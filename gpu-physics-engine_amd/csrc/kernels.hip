@@ -1,0 +1,999 @@
+// kernels.hip — gfx950 kernels of the MLS-MPM substep.
+//
+// Reference kernels (libs/cuda-engine/source/MPM.cu) and their counterparts here:
+//   resetGridKernel   :46-51   -> folded into k_grid_update (it zeroes the accumulators it consumed)
+//   P2GKernel         :53-129  -> k_p2g_reference (same structure: one particle per lane, 27 HBM atomics)
+//                                 k_p2g_tiled     (binned particles, LDS tile, one HBM atomic per tile node)
+//   gridKernel        :131-177 -> k_grid_update
+//   G2PKernel         :178-301 -> k_g2p_reference / k_g2p_tiled (LDS-staged velocity tile)
+//   (P2G of s+1 fused with G2P of s, no reference counterpart) -> k_g2p2g
+//   MPMOutputKernel   :302-322 -> k_output
+//   MPMEmitterKernel  :380-438 -> k_emitter (own Philox4x32-10 + Box-Muller, wave64 ballot)
+// plus the stable counting/radix binning that the reference MPM does not have (the reference's only
+// sort, PBF.cu:738-834, ranks with atomicAdd and is order-nondeterministic; this one is stable).
+#include "kernels.h"
+
+#include <hip/hip_runtime.h>
+
+namespace mpm {
+
+// ===================================================================================== small utilities
+
+__global__ void k_fill_f32(float* p, size_t n, float v)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) { p[i] = v; }
+}
+__global__ void k_fill_i32(int32_t* p, size_t n, int32_t v)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) { p[i] = v; }
+}
+
+static unsigned fill_blocks(size_t n)
+{
+    size_t b = (n + kThreads - 1) / kThreads;
+    if (b > 65536) { b = 65536; }
+    return (unsigned)(b ? b : 1);
+}
+
+hipError_t launch_fill_f32(hipStream_t st, float* p, size_t n, float v)
+{
+    if (n == 0) { return hipSuccess; }
+    hipLaunchKernelGGL(k_fill_f32, dim3(fill_blocks(n)), dim3(kThreads), 0, st, p, n, v);
+    return hipGetLastError();
+}
+hipError_t launch_fill_i32(hipStream_t st, int32_t* p, size_t n, int32_t v)
+{
+    if (n == 0) { return hipSuccess; }
+    hipLaunchKernelGGL(k_fill_i32, dim3(fill_blocks(n)), dim3(kThreads), 0, st, p, n, v);
+    return hipGetLastError();
+}
+
+__device__ __forceinline__ int scene_of_key(const DevParams& p, uint32_t key)
+{
+    return p.num_scenes > 1 ? (int)(key / (uint32_t)p.bps) : 0;
+}
+
+__device__ __forceinline__ void atomic_add_global(float* a, float v) { unsafeAtomicAdd(a, v); }
+__device__ __forceinline__ void atomic_add_lds(float* a, float v)
+{
+    __hip_atomic_fetch_add(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// ===================================================================================== grid update
+
+// gridKernel (MPM.cu:131-177) for every node of every scene, plus the reset of the accumulators it
+// consumed (resetGridKernel, MPM.cu:46-51), so the next P2G starts from zero without a separate pass.
+// Each lane handles 4 consecutive nodes (float4 loads of the accumulator planes).
+__global__ __launch_bounds__(kThreads) void k_grid_update(DevParams p, Grid g, size_t total)
+{
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t i0 = q * 4;
+    if (i0 >= total) { return; }
+    if (i0 + 4 <= total) {
+        float4 mx = *reinterpret_cast<const float4*>(g.mom_x + i0);
+        float4 my = *reinterpret_cast<const float4*>(g.mom_y + i0);
+        float4 m = *reinterpret_cast<const float4*>(g.mass + i0);
+        const float amx[4] = {mx.x, mx.y, mx.z, mx.w};
+        const float amy[4] = {my.x, my.y, my.z, my.w};
+        const float am[4] = {m.x, m.y, m.z, m.w};
+        float2 out[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const size_t idx = i0 + k;
+            const int local = (int)(idx % (size_t)p.G);
+            const int y = local / p.W;
+            const int x = local - y * p.W;
+            out[k] = node_velocity(p, amx[k], amy[k], am[k], x, y);
+        }
+        float4* vel4 = reinterpret_cast<float4*>(g.vel + i0);
+        vel4[0] = make_float4(out[0].x, out[0].y, out[1].x, out[1].y);
+        vel4[1] = make_float4(out[2].x, out[2].y, out[3].x, out[3].y);
+        *reinterpret_cast<float4*>(g.vmass + i0) = m;
+        const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+        *reinterpret_cast<float4*>(g.mom_x + i0) = z;
+        *reinterpret_cast<float4*>(g.mom_y + i0) = z;
+        *reinterpret_cast<float4*>(g.mass + i0) = z;
+    } else {
+        for (size_t idx = i0; idx < total; ++idx) {
+            const int local = (int)(idx % (size_t)p.G);
+            const int y = local / p.W;
+            const int x = local - y * p.W;
+            g.vel[idx] = node_velocity(p, g.mom_x[idx], g.mom_y[idx], g.mass[idx], x, y);
+            g.vmass[idx] = g.mass[idx];
+            g.mom_x[idx] = 0.f;
+            g.mom_y[idx] = 0.f;
+            g.mass[idx] = 0.f;
+        }
+    }
+}
+
+hipError_t launch_grid_update(hipStream_t st, const DevParams& p, const Grid& g)
+{
+    const size_t total = (size_t)p.G * (size_t)p.num_scenes;
+    const size_t lanes = (total + 3) / 4;
+    hipLaunchKernelGGL(k_grid_update, dim3((unsigned)((lanes + kThreads - 1) / kThreads)), dim3(kThreads), 0, st, p,
+                       g, total);
+    return hipGetLastError();
+}
+
+// ===================================================================================== per-particle helpers
+
+__device__ __forceinline__ Mat2 load_F(const Store& s, int i) { return {s.f00[i], s.f01[i], s.f10[i], s.f11[i]}; }
+__device__ __forceinline__ Mat2 load_C(const Store& s, int i) { return {s.c00[i], s.c01[i], s.c10[i], s.c11[i]}; }
+__device__ __forceinline__ void store_F(const Store& s, int i, const Mat2& F)
+{
+    s.f00[i] = F.a00;
+    s.f01[i] = F.a01;
+    s.f10[i] = F.a10;
+    s.f11[i] = F.a11;
+}
+__device__ __forceinline__ void store_C(const Store& s, int i, const Mat2& C)
+{
+    s.c00[i] = C.a00;
+    s.c01[i] = C.a01;
+    s.c10[i] = C.a10;
+    s.c11[i] = C.a11;
+}
+
+// ===================================================================================== reference structure
+
+// P2GKernel + particleToGrid (MPM.cu:53-129): one particle per lane, 27 float atomics into HBM.
+__global__ __launch_bounds__(kThreads) void k_p2g_reference(DevParams p, Store s, const int32_t* count, Grid g)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count[0]) { return; }
+    const int scene = scene_of_key(p, s.key[i]);
+    const size_t gb = (size_t)scene * (size_t)p.G;
+    const int mat = s.mat[i];
+    const Mat2 Q = affine_q(p, mat, load_C(s, i), load_F(s, i), s.J[i]);
+    const float vx = s.vx[i], vy = s.vy[i];
+    int bx, by;
+    float fx, fy, wx[3], wy[3];
+    bspline(s.x[i], p.inv_dx, (float)p.halo, bx, fx, wx);
+    bspline(s.y[i], p.inv_dx, (float)p.halo, by, fy, wy);
+    bx = clampi(bx, 0, p.W - 3);
+    by = clampi(by, 0, p.H - 3);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            const float w = wx[a] * wy[b];
+            const float dpx = ((float)a - fx) * p.dx;
+            const float dpy = ((float)b - fy) * p.dx;
+            const float afx = dpx * Q.a00 + dpy * Q.a01;
+            const float afy = dpx * Q.a10 + dpy * Q.a11;
+            const size_t node = gb + (size_t)(by + b) * (size_t)p.W + (size_t)(bx + a);
+            atomic_add_global(&g.mom_x[node], w * (vx * p.mass + afx));
+            atomic_add_global(&g.mom_y[node], w * (vy * p.mass + afy));
+            atomic_add_global(&g.mass[node], w * p.mass);
+        }
+    }
+}
+
+// G2PKernel + gridToParticle (MPM.cu:178-266): one particle per lane, 9 velocity gathers from HBM.
+__global__ __launch_bounds__(kThreads) void k_g2p_reference(DevParams p, Store s, const int32_t* count, Grid g)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count[0]) { return; }
+    const int scene = scene_of_key(p, s.key[i]);
+    const size_t gb = (size_t)scene * (size_t)p.G;
+    float px = s.x[i], py = s.y[i];
+    int bx, by;
+    float fx, fy, wx[3], wy[3];
+    bspline(px, p.inv_dx, (float)p.halo, bx, fx, wx);
+    bspline(py, p.inv_dx, (float)p.halo, by, fy, wy);
+    bx = clampi(bx, 0, p.W - 3);
+    by = clampi(by, 0, p.H - 3);
+    float vx = 0.f, vy = 0.f;
+    Mat2 B = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            const float w = wx[a] * wy[b];
+            const float dpx = ((float)a - fx) * p.dx;
+            const float dpy = ((float)b - fy) * p.dx;
+            const float2 gv = g.vel[gb + (size_t)(by + b) * (size_t)p.W + (size_t)(bx + a)];
+            vx += w * gv.x;
+            vy += w * gv.y;
+            const float wvx = w * gv.x, wvy = w * gv.y;
+            B.a00 += wvx * dpx;
+            B.a01 += wvx * dpy;
+            B.a10 += wvy * dpx;
+            B.a11 += wvy * dpy;
+        }
+    }
+    Mat2 C, F = load_F(s, i);
+    float J = s.J[i];
+    g2p_update(p, s.mat[i], px, py, vx, vy, B, C, F, J);
+    s.x[i] = px;
+    s.y[i] = py;
+    s.vx[i] = vx;
+    s.vy[i] = vy;
+    store_C(s, i, C);
+    store_F(s, i, F);
+    s.J[i] = J;
+}
+
+hipError_t launch_p2g_reference(hipStream_t st, const DevParams& p, const Store& s, const int32_t* count, int cap,
+                                const Grid& g)
+{
+    hipLaunchKernelGGL(k_p2g_reference, dim3((unsigned)((cap + kThreads - 1) / kThreads)), dim3(kThreads), 0, st, p,
+                       s, count, g);
+    return hipGetLastError();
+}
+hipError_t launch_g2p_reference(hipStream_t st, const DevParams& p, const Store& s, const int32_t* count, int cap,
+                                const Grid& g)
+{
+    hipLaunchKernelGGL(k_g2p_reference, dim3((unsigned)((cap + kThreads - 1) / kThreads)), dim3(kThreads), 0, st, p,
+                       s, count, g);
+    return hipGetLastError();
+}
+
+// ===================================================================================== binning (stable)
+
+// Bin key of every particle from its position; the scene comes from the previous key.
+__global__ __launch_bounds__(kThreads) void k_bin_keys(DevParams p, Store s, const int32_t* count, uint32_t* keys)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count[0]) { return; }
+    const uint32_t scene = p.num_scenes > 1 ? s.key[i] / (uint32_t)p.bps : 0u;
+    keys[i] = bin_key(p, s.x[i], s.y[i], scene);
+}
+
+// Radix upsweep: digit histogram of one tile of kRadixTile keys, written digit-major hist[d * ntiles + t].
+__global__ __launch_bounds__(kThreads) void k_radix_upsweep(const uint32_t* keys, const int32_t* count, int shift,
+                                                            int32_t* hist, int ntiles)
+{
+    __shared__ int32_t lh[kRadixBins];
+    const int t = blockIdx.x;
+    const int n = count[0];
+    for (int d = threadIdx.x; d < kRadixBins; d += kThreads) { lh[d] = 0; }
+    __syncthreads();
+    const int base = t * kRadixTile;
+    if (base < n) {
+#pragma unroll 4
+        for (int r = 0; r < kRadixItems; ++r) {
+            const int idx = base + r * kThreads + threadIdx.x;
+            if (idx < n) { atomicAdd(&lh[(keys[idx] >> shift) & (kRadixBins - 1)], 1); }
+        }
+    }
+    __syncthreads();
+    for (int d = threadIdx.x; d < kRadixBins; d += kThreads) { hist[d * ntiles + t] = lh[d]; }
+}
+
+// Work-group exclusive scan of kThreads values; returns the prefix of the calling lane, *total the sum.
+__device__ __forceinline__ int block_exclusive_scan(int v, int* total)
+{
+    __shared__ int wsum[kThreads / 64];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o) { x += y; }
+    }
+    if (lane == 63) { wsum[wave] = x; }
+    __syncthreads();
+    int wpre = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < kThreads / 64; ++w) {
+        const int c = wsum[w];
+        if (w < wave) { wpre += c; }
+        tot += c;
+    }
+    __syncthreads();
+    *total = tot;
+    return wpre + x - v;
+}
+
+constexpr int kScanItems = 16;
+constexpr int kScanSeg = kThreads * kScanItems;
+
+// Scan phase 1: total of each segment of kScanSeg values.
+__global__ __launch_bounds__(kThreads) void k_scan_partials(const int32_t* data, int len, int32_t* partials)
+{
+    const int base = blockIdx.x * kScanSeg + threadIdx.x * kScanItems;
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const int i = base + k;
+        if (i < len) { s += data[i]; }
+    }
+    int total;
+    block_exclusive_scan(s, &total);
+    if (threadIdx.x == 0) { partials[blockIdx.x] = total; }
+}
+// Scan phase 2: exclusive scan of the segment totals by one work group (any count).
+__global__ __launch_bounds__(kThreads) void k_scan_single(int32_t* partials, int nseg)
+{
+    int carry = 0;
+    for (int base = 0; base < nseg; base += kThreads) {
+        const int i = base + threadIdx.x;
+        const int v = i < nseg ? partials[i] : 0;
+        int total;
+        const int pre = block_exclusive_scan(v, &total);
+        if (i < nseg) { partials[i] = carry + pre; }
+        carry += total;
+        __syncthreads();
+    }
+}
+// Scan phase 3: exclusive scan of each segment in place, offset by its segment prefix.
+__global__ __launch_bounds__(kThreads) void k_scan_apply(int32_t* data, int len, const int32_t* partials)
+{
+    const int base = blockIdx.x * kScanSeg + threadIdx.x * kScanItems;
+    int vals[kScanItems];
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const int i = base + k;
+        vals[k] = i < len ? data[i] : 0;
+        s += vals[k];
+    }
+    int total;
+    int run = block_exclusive_scan(s, &total) + partials[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const int i = base + k;
+        if (i < len) { data[i] = run; }
+        run += vals[k];
+    }
+}
+
+static hipError_t exclusive_scan(hipStream_t st, int32_t* data, int len, int32_t* partials)
+{
+    const int nseg = (len + kScanSeg - 1) / kScanSeg;
+    if (nseg == 0) { return hipSuccess; }
+    hipLaunchKernelGGL(k_scan_partials, dim3(nseg), dim3(kThreads), 0, st, data, len, partials);
+    hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(kThreads), 0, st, partials, nseg);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nseg), dim3(kThreads), 0, st, data, len, partials);
+    return hipGetLastError();
+}
+
+// Radix downsweep: stable scatter of one tile.  The tile is consumed in kRadixItems rounds of kThreads
+// consecutive keys; inside a round a wave ranks its lanes by digit with 8 ballots (lanes with the same
+// digit and a lower lane id come first), the waves of the round are ordered by a per-digit prefix over
+// wave counts, and a running per-digit count carries the order across rounds.  Ranks therefore follow
+// the input index order exactly: the sort is stable and deterministic.
+__global__ __launch_bounds__(kThreads) void k_radix_downsweep(const uint32_t* keys_in, const int32_t* vals_in,
+                                                              const int32_t* count, int shift, const int32_t* offsets,
+                                                              int ntiles, uint32_t* keys_out, int32_t* vals_out)
+{
+    constexpr int kWaves = kThreads / 64;
+    __shared__ int32_t wcnt[kWaves][kRadixBins];
+    __shared__ int32_t wpre[kWaves][kRadixBins];
+    __shared__ int32_t run[kRadixBins];
+    const int t = blockIdx.x;
+    const int n = count[0];
+    const int base = t * kRadixTile;
+    if (base >= n) { return; }
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int d = threadIdx.x; d < kRadixBins; d += kThreads) {
+        run[d] = offsets[d * ntiles + t];
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) { wcnt[w][d] = 0; }
+    }
+    __syncthreads();
+    for (int r = 0; r < kRadixItems; ++r) {
+        const int idx = base + r * kThreads + threadIdx.x;
+        const bool valid = idx < n;
+        uint32_t key = 0;
+        int32_t val = 0;
+        if (valid) {
+            key = keys_in[idx];
+            val = vals_in ? vals_in[idx] : idx;
+        }
+        const uint32_t digit = (key >> shift) & (kRadixBins - 1);
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < kRadixBits; ++b) {
+            const uint64_t m = __ballot((digit >> b) & 1u);
+            peers &= ((digit >> b) & 1u) ? m : ~m;
+        }
+        const int rank = __popcll(peers & lt_mask);
+        if (valid && (peers & lt_mask) == 0) { wcnt[wave][digit] = __popcll(peers); }
+        __syncthreads();
+        for (int d = threadIdx.x; d < kRadixBins; d += kThreads) {
+            int acc = run[d];
+#pragma unroll
+            for (int w = 0; w < kWaves; ++w) {
+                wpre[w][d] = acc;
+                acc += wcnt[w][d];
+                wcnt[w][d] = 0;
+            }
+            run[d] = acc;
+        }
+        __syncthreads();
+        if (valid) {
+            const int pos = wpre[wave][digit] + rank;
+            if ((unsigned)pos < (unsigned)n) {  // always true for a consistent histogram
+                keys_out[pos] = key;
+                vals_out[pos] = val;
+            }
+        }
+    }
+}
+
+// Permutation gather of the whole particle state: dst[j] = src[perm[j]] (keys already in dst.key).
+__global__ __launch_bounds__(kThreads) void k_gather(Store src, Store dst, const int32_t* perm, const int32_t* count)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = count[0];
+    if (j >= n) { return; }
+    const int i = perm[j];
+    if ((unsigned)i >= (unsigned)n) { return; }
+    dst.x[j] = src.x[i];
+    dst.y[j] = src.y[i];
+    dst.vx[j] = src.vx[i];
+    dst.vy[j] = src.vy[i];
+    dst.c00[j] = src.c00[i];
+    dst.c01[j] = src.c01[i];
+    dst.c10[j] = src.c10[i];
+    dst.c11[j] = src.c11[i];
+    dst.f00[j] = src.f00[i];
+    dst.f01[j] = src.f01[i];
+    dst.f10[j] = src.f10[i];
+    dst.f11[j] = src.f11[i];
+    dst.J[j] = src.J[i];
+    dst.mat[j] = src.mat[i];
+    dst.id[j] = src.id[i];
+}
+
+// Bin ranges from the sorted keys (bins not present keep start = end = 0).
+__global__ __launch_bounds__(kThreads) void k_bin_bounds(const uint32_t* keys, const int32_t* count, int nkeys,
+                                                         int32_t* bstart, int32_t* bend)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = count[0];
+    if (j >= n) { return; }
+    const uint32_t k = keys[j];
+    if (k >= (uint32_t)nkeys) { return; }
+    if (j == 0 || keys[j - 1] != k) { bstart[k] = j; }
+    if (j == n - 1 || keys[j + 1] != k) { bend[k] = j + 1; }
+}
+// Work items per bin: ceil(count / chunk).
+__global__ __launch_bounds__(kThreads) void k_item_counts(const int32_t* bstart, const int32_t* bend, int nkeys,
+                                                          int chunk, int32_t* cnt)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nkeys) { return; }
+    const int c = bend[k] - bstart[k];
+    cnt[k] = (c + chunk - 1) / chunk;
+}
+// Work item list from the scanned item counts; the last bin also publishes the total.
+__global__ __launch_bounds__(kThreads) void k_items_write(const int32_t* bstart, const int32_t* bend,
+                                                          const int32_t* off, int nkeys, int chunk, Item* items,
+                                                          int32_t* num_items)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nkeys) { return; }
+    const int b = bstart[k], e = bend[k];
+    const int c = (e - b + chunk - 1) / chunk;
+    const int o = off[k];
+    for (int q = 0; q < c; ++q) {
+        Item it;
+        it.key = k;
+        it.begin = b + q * chunk;
+        it.end = min(b + (q + 1) * chunk, e);
+        it.pad = 0;
+        items[o + q] = it;
+    }
+    if (k == nkeys - 1) { num_items[0] = o + c; }
+}
+
+hipError_t launch_binning(hipStream_t st, const DevParams& p, const Store& src, const Store& dst,
+                          const int32_t* count, const SortScratch& ss)
+{
+    const int cap = ss.capacity;
+    const unsigned pblocks = (unsigned)((cap + kThreads - 1) / kThreads);
+    hipLaunchKernelGGL(k_bin_keys, dim3(pblocks), dim3(kThreads), 0, st, p, src, count, ss.keys_tmp);
+    const int passes = (ss.key_bits + kRadixBits - 1) / kRadixBits;
+    const uint32_t* kin = ss.keys_tmp;
+    const int32_t* vin = nullptr;
+    for (int pass = 0; pass < passes; ++pass) {
+        const bool last = pass == passes - 1;
+        uint32_t* kout = last ? dst.key : (pass % 2 == 0 ? ss.keys_alt : ss.keys_tmp);
+        int32_t* vout = last ? ss.perm : (pass % 2 == 0 ? ss.vals_alt : ss.vals_tmp);
+        const int shift = pass * kRadixBits;
+        hipLaunchKernelGGL(k_radix_upsweep, dim3(ss.ntiles), dim3(kThreads), 0, st, kin, count, shift, ss.hist,
+                           ss.ntiles);
+        hipError_t e = exclusive_scan(st, ss.hist, kRadixBins * ss.ntiles, ss.partials);
+        if (e != hipSuccess) { return e; }
+        hipLaunchKernelGGL(k_radix_downsweep, dim3(ss.ntiles), dim3(kThreads), 0, st, kin, vin, count, shift,
+                           ss.hist, ss.ntiles, kout, vout);
+        kin = kout;
+        vin = vout;
+    }
+    if (passes == 0) { return hipErrorInvalidValue; }
+    hipLaunchKernelGGL(k_gather, dim3(pblocks), dim3(kThreads), 0, st, src, dst, ss.perm, count);
+    hipError_t e = launch_fill_i32(st, ss.bin_start, (size_t)ss.num_keys, 0);
+    if (e != hipSuccess) { return e; }
+    e = launch_fill_i32(st, ss.bin_end, (size_t)ss.num_keys, 0);
+    if (e != hipSuccess) { return e; }
+    hipLaunchKernelGGL(k_bin_bounds, dim3(pblocks), dim3(kThreads), 0, st, dst.key, count, ss.num_keys, ss.bin_start,
+                       ss.bin_end);
+    const unsigned kblocks = (unsigned)((ss.num_keys + kThreads - 1) / kThreads);
+    hipLaunchKernelGGL(k_item_counts, dim3(kblocks), dim3(kThreads), 0, st, ss.bin_start, ss.bin_end, ss.num_keys,
+                       ss.chunk, ss.item_cnt);
+    e = exclusive_scan(st, ss.item_cnt, ss.num_keys, ss.partials);
+    if (e != hipSuccess) { return e; }
+    hipLaunchKernelGGL(k_items_write, dim3(kblocks), dim3(kThreads), 0, st, ss.bin_start, ss.bin_end, ss.item_cnt,
+                       ss.num_keys, ss.chunk, ss.items, ss.num_items);
+    return hipGetLastError();
+}
+
+// ===================================================================================== binned kernels
+
+// Work groups are dispatched round-robin over the 8 XCDs; give each XCD a contiguous run of work items
+// (neighbouring bins) so the velocity tiles and accumulator flushes of neighbours share that XCD's L2.
+__device__ __forceinline__ int xcd_item(int num_items, bool& ok)
+{
+    const int per = (num_items + 7) >> 3;
+    const int xcd = blockIdx.x & 7;
+    const int slot = blockIdx.x >> 3;
+    const int it = xcd * per + slot;
+    ok = slot < per && it < num_items;
+    return it;
+}
+
+template <int BX, int BY>
+struct TileShape {
+    static constexpr int TX = BX + 2 + 2 * kMargin;
+    static constexpr int TY = BY + 2 + 2 * kMargin;
+    static constexpr int N = TX * TY;
+};
+
+__device__ __forceinline__ void item_origin(const DevParams& p, int key, int BX, int BY, int& scene, int& ox, int& oy)
+{
+    scene = key / p.bps;
+    const int r = key - scene * p.bps;
+    const int by = r / p.nbx;
+    const int bx = r - by * p.nbx;
+    ox = bx * BX - kMargin;
+    oy = by * BY - kMargin;
+}
+
+template <int BX, int BY>
+__device__ __forceinline__ void load_vel_tile(const DevParams& p, const Grid& g, size_t gb, int ox, int oy, float2* sv)
+{
+    using T = TileShape<BX, BY>;
+    for (int t = threadIdx.x; t < T::N; t += kThreads) {
+        const int ty = t / T::TX;
+        const int tx = t - ty * T::TX;
+        const int gx = ox + tx, gy = oy + ty;
+        float2 v = {0.f, 0.f};
+        if (gx >= 0 && gx < p.W && gy >= 0 && gy < p.H) { v = g.vel[gb + (size_t)gy * p.W + gx]; }
+        sv[t] = v;
+    }
+}
+
+template <int BX, int BY>
+__device__ __forceinline__ void zero_acc_tile(float* sax, float* say, float* sam)
+{
+    using T = TileShape<BX, BY>;
+    for (int t = threadIdx.x; t < T::N; t += kThreads) {
+        sax[t] = 0.f;
+        say[t] = 0.f;
+        sam[t] = 0.f;
+    }
+}
+
+// One HBM atomic per touched tile node (the LDS tile already reduced every particle of the item).
+template <int BX, int BY>
+__device__ __forceinline__ void flush_acc_tile(const DevParams& p, const Grid& g, size_t gb, int ox, int oy,
+                                               const float* sax, const float* say, const float* sam)
+{
+    using T = TileShape<BX, BY>;
+    for (int t = threadIdx.x; t < T::N; t += kThreads) {
+        const float m = sam[t];
+        const float mx = sax[t];
+        const float my = say[t];
+        if (m == 0.f && mx == 0.f && my == 0.f) { continue; }
+        const int ty = t / T::TX;
+        const int tx = t - ty * T::TX;
+        const int gx = ox + tx, gy = oy + ty;
+        if (gx < 0 || gx >= p.W || gy < 0 || gy >= p.H) { continue; }
+        const size_t node = gb + (size_t)gy * p.W + gx;
+        atomic_add_global(&g.mom_x[node], mx);
+        atomic_add_global(&g.mom_y[node], my);
+        atomic_add_global(&g.mass[node], m);
+    }
+}
+
+// Gather of one particle from the LDS velocity tile (or HBM when it drifted out of the tile).
+template <int BX, int BY>
+__device__ __forceinline__ void gather(const DevParams& p, const Grid& g, size_t gb, int ox, int oy, const float2* sv,
+                                       float px, float py, float& vx, float& vy, Mat2& B)
+{
+    using T = TileShape<BX, BY>;
+    int bx, by;
+    float fx, fy, wx[3], wy[3];
+    bspline(px, p.inv_dx, (float)p.halo, bx, fx, wx);
+    bspline(py, p.inv_dx, (float)p.halo, by, fy, wy);
+    bx = clampi(bx, 0, p.W - 3);
+    by = clampi(by, 0, p.H - 3);
+    const int lx = bx - ox, ly = by - oy;
+    const bool inside = (unsigned)lx <= (unsigned)(T::TX - 3) && (unsigned)ly <= (unsigned)(T::TY - 3);
+    vx = 0.f;
+    vy = 0.f;
+    B = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            const float w = wx[a] * wy[b];
+            const float dpx = ((float)a - fx) * p.dx;
+            const float dpy = ((float)b - fy) * p.dx;
+            float2 gv;
+            if (inside) {
+                gv = sv[(ly + b) * T::TX + lx + a];
+            } else {
+                gv = g.vel[gb + (size_t)(by + b) * p.W + (bx + a)];
+            }
+            vx += w * gv.x;
+            vy += w * gv.y;
+            const float wvx = w * gv.x, wvy = w * gv.y;
+            B.a00 += wvx * dpx;
+            B.a01 += wvx * dpy;
+            B.a10 += wvy * dpx;
+            B.a11 += wvy * dpy;
+        }
+    }
+}
+
+// Scatter of one particle into the LDS accumulator tile (or HBM when it drifted out of the tile).
+template <int BX, int BY>
+__device__ __forceinline__ void scatter(const DevParams& p, const Grid& g, size_t gb, int ox, int oy, float* sax,
+                                        float* say, float* sam, float px, float py, float vx, float vy, const Mat2& Q)
+{
+    using T = TileShape<BX, BY>;
+    int bx, by;
+    float fx, fy, wx[3], wy[3];
+    bspline(px, p.inv_dx, (float)p.halo, bx, fx, wx);
+    bspline(py, p.inv_dx, (float)p.halo, by, fy, wy);
+    bx = clampi(bx, 0, p.W - 3);
+    by = clampi(by, 0, p.H - 3);
+    const int lx = bx - ox, ly = by - oy;
+    const bool inside = (unsigned)lx <= (unsigned)(T::TX - 3) && (unsigned)ly <= (unsigned)(T::TY - 3);
+    const float mvx = vx * p.mass, mvy = vy * p.mass;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            const float w = wx[a] * wy[b];
+            const float dpx = ((float)a - fx) * p.dx;
+            const float dpy = ((float)b - fy) * p.dx;
+            const float afx = dpx * Q.a00 + dpy * Q.a01;
+            const float afy = dpx * Q.a10 + dpy * Q.a11;
+            const float ax = w * (mvx + afx), ay = w * (mvy + afy), am = w * p.mass;
+            if (inside) {
+                const int t = (ly + b) * T::TX + lx + a;
+                atomic_add_lds(&sax[t], ax);
+                atomic_add_lds(&say[t], ay);
+                atomic_add_lds(&sam[t], am);
+            } else {
+                const size_t node = gb + (size_t)(by + b) * p.W + (bx + a);
+                atomic_add_global(&g.mom_x[node], ax);
+                atomic_add_global(&g.mom_y[node], ay);
+                atomic_add_global(&g.mass[node], am);
+            }
+        }
+    }
+}
+
+// P2G over binned particles: one work item (<= chunk particles of one bin) per work group.
+template <int BX, int BY>
+__global__ __launch_bounds__(kThreads) void k_p2g_tiled(DevParams p, Store s, Grid g, const Item* items,
+                                                        const int32_t* num_items, const int32_t* count)
+{
+    using T = TileShape<BX, BY>;
+    __shared__ float sax[T::N], say[T::N], sam[T::N];
+    bool ok;
+    const int it = xcd_item(num_items[0], ok);
+    if (!ok) { return; }
+    Item item = items[it];
+    item.end = min(item.end, count[0]);
+    int scene, ox, oy;
+    item_origin(p, item.key, BX, BY, scene, ox, oy);
+    const size_t gb = (size_t)scene * (size_t)p.G;
+    zero_acc_tile<BX, BY>(sax, say, sam);
+    __syncthreads();
+    for (int j = item.begin + threadIdx.x; j < item.end; j += kThreads) {
+        const Mat2 Q = affine_q(p, s.mat[j], load_C(s, j), load_F(s, j), s.J[j]);
+        scatter<BX, BY>(p, g, gb, ox, oy, sax, say, sam, s.x[j], s.y[j], s.vx[j], s.vy[j], Q);
+    }
+    __syncthreads();
+    flush_acc_tile<BX, BY>(p, g, gb, ox, oy, sax, say, sam);
+}
+
+// G2P over binned particles, velocities staged in an LDS tile.
+template <int BX, int BY>
+__global__ __launch_bounds__(kThreads) void k_g2p_tiled(DevParams p, Store s, Grid g, const Item* items,
+                                                        const int32_t* num_items, const int32_t* count)
+{
+    using T = TileShape<BX, BY>;
+    __shared__ float2 sv[T::N];
+    bool ok;
+    const int it = xcd_item(num_items[0], ok);
+    if (!ok) { return; }
+    Item item = items[it];
+    item.end = min(item.end, count[0]);
+    int scene, ox, oy;
+    item_origin(p, item.key, BX, BY, scene, ox, oy);
+    const size_t gb = (size_t)scene * (size_t)p.G;
+    load_vel_tile<BX, BY>(p, g, gb, ox, oy, sv);
+    __syncthreads();
+    for (int j = item.begin + threadIdx.x; j < item.end; j += kThreads) {
+        float px = s.x[j], py = s.y[j], vx, vy;
+        Mat2 B;
+        gather<BX, BY>(p, g, gb, ox, oy, sv, px, py, vx, vy, B);
+        Mat2 C, F = load_F(s, j);
+        float J = s.J[j];
+        g2p_update(p, s.mat[j], px, py, vx, vy, B, C, F, J);
+        s.x[j] = px;
+        s.y[j] = py;
+        s.vx[j] = vx;
+        s.vy[j] = vy;
+        store_C(s, j, C);
+        store_F(s, j, F);
+        s.J[j] = J;
+    }
+}
+
+// G2P of substep s fused with P2G of substep s+1: the particle is read once (x, F, J, material), gathers
+// from the LDS velocity tile of grid s, is updated, and scatters into the LDS accumulator tile of grid
+// s+1, flushed with one HBM atomic per touched node.  v and C stay in registers (they are recomputed
+// by every G2P); they are written only by the last G2P of a segment (k_g2p_tiled).
+template <int BX, int BY>
+__global__ __launch_bounds__(kThreads) void k_g2p2g(DevParams p, Store s, Grid g, const Item* items,
+                                                    const int32_t* num_items, const int32_t* count)
+{
+    using T = TileShape<BX, BY>;
+    __shared__ float2 sv[T::N];
+    __shared__ float sax[T::N], say[T::N], sam[T::N];
+    bool ok;
+    const int it = xcd_item(num_items[0], ok);
+    if (!ok) { return; }
+    Item item = items[it];
+    item.end = min(item.end, count[0]);
+    int scene, ox, oy;
+    item_origin(p, item.key, BX, BY, scene, ox, oy);
+    const size_t gb = (size_t)scene * (size_t)p.G;
+    load_vel_tile<BX, BY>(p, g, gb, ox, oy, sv);
+    zero_acc_tile<BX, BY>(sax, say, sam);
+    __syncthreads();
+    for (int j = item.begin + threadIdx.x; j < item.end; j += kThreads) {
+        float px = s.x[j], py = s.y[j], vx, vy;
+        const int mat = s.mat[j];
+        Mat2 B;
+        gather<BX, BY>(p, g, gb, ox, oy, sv, px, py, vx, vy, B);
+        Mat2 C, F = load_F(s, j);
+        float J = s.J[j];
+        g2p_update(p, mat, px, py, vx, vy, B, C, F, J);
+        s.x[j] = px;
+        s.y[j] = py;
+        store_F(s, j, F);
+        s.J[j] = J;
+        const Mat2 Q = affine_q(p, mat, C, F, J);
+        scatter<BX, BY>(p, g, gb, ox, oy, sax, say, sam, px, py, vx, vy, Q);
+    }
+    __syncthreads();
+    flush_acc_tile<BX, BY>(p, g, gb, ox, oy, sax, say, sam);
+}
+
+int shape_index(int bx, int by)
+{
+    for (int k = 0; k < kNumShapes; ++k) {
+        if (kShapeX[k] == bx && kShapeY[k] == by) { return k; }
+    }
+    return -1;
+}
+
+static unsigned item_blocks(const SortScratch& ss) { return (unsigned)(((ss.max_items + 7) / 8) * 8); }
+
+#define MPM_DISPATCH_SHAPE(KERNEL)                                                                              \
+    switch (shape) {                                                                                            \
+    case 0: hipLaunchKernelGGL((KERNEL<8, 8>), dim3(nb), dim3(kThreads), 0, st, p, s, g, ss.items, ss.num_items, count); \
+        break;                                                                                                  \
+    case 1: hipLaunchKernelGGL((KERNEL<16, 8>), dim3(nb), dim3(kThreads), 0, st, p, s, g, ss.items, ss.num_items, count); \
+        break;                                                                                                  \
+    case 2: hipLaunchKernelGGL((KERNEL<16, 16>), dim3(nb), dim3(kThreads), 0, st, p, s, g, ss.items, ss.num_items, count); \
+        break;                                                                                                  \
+    case 3: hipLaunchKernelGGL((KERNEL<32, 16>), dim3(nb), dim3(kThreads), 0, st, p, s, g, ss.items, ss.num_items, count); \
+        break;                                                                                                  \
+    default: return hipErrorInvalidValue;                                                                       \
+    }
+
+hipError_t launch_p2g_tiled(hipStream_t st, int shape, const DevParams& p, const Store& s, const Grid& g,
+                            const SortScratch& ss, const int32_t* count)
+{
+    const unsigned nb = item_blocks(ss);
+    MPM_DISPATCH_SHAPE(k_p2g_tiled)
+    return hipGetLastError();
+}
+hipError_t launch_g2p_tiled(hipStream_t st, int shape, const DevParams& p, const Store& s, const Grid& g,
+                            const SortScratch& ss, const int32_t* count)
+{
+    const unsigned nb = item_blocks(ss);
+    MPM_DISPATCH_SHAPE(k_g2p_tiled)
+    return hipGetLastError();
+}
+hipError_t launch_g2p2g(hipStream_t st, int shape, const DevParams& p, const Store& s, const Grid& g,
+                        const SortScratch& ss, const int32_t* count)
+{
+    const unsigned nb = item_blocks(ss);
+    MPM_DISPATCH_SHAPE(k_g2p2g)
+    return hipGetLastError();
+}
+
+// ===================================================================================== output
+
+__device__ OutView g_out_view;
+
+// MPMOutputKernel (MPM.cu:302-322): positions and velocities into the current output slot, in the
+// caller's particle order, with per-chunk particle counts; count[1] = count[0] afterwards.
+__global__ __launch_bounds__(kThreads) void k_output(Store s, int32_t* count)
+{
+    const int n = count[0];
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) { return; }
+    const OutView ov = g_out_view;
+    const int i = s.id[j];
+    if ((unsigned)i >= (unsigned)n) { return; }
+    reinterpret_cast<float2*>(ov.pos)[i] = make_float2(s.x[j], s.y[j]);
+    reinterpret_cast<float2*>(ov.vel)[i] = make_float2(s.vx[j], s.vy[j]);
+    const int oc = ov.out_chunk;
+    const int chunk = i / oc;
+    if (i + 1 == n) {
+        ov.num_atoms[chunk] = (i % oc) + 1;
+        count[1] = n;
+    } else if ((i % oc) + 1 == oc) {
+        ov.num_atoms[chunk] = oc;
+    }
+}
+
+hipError_t launch_output(hipStream_t st, const Store& s, int32_t* count, int cap)
+{
+    hipLaunchKernelGGL(k_output, dim3((unsigned)((cap + kThreads - 1) / kThreads)), dim3(kThreads), 0, st, s, count);
+    return hipGetLastError();
+}
+
+hipError_t set_output_view_async(hipStream_t st, const OutView* host_pinned)
+{
+    return hipMemcpyToSymbolAsync(HIP_SYMBOL(g_out_view), host_pinned, sizeof(OutView), 0, hipMemcpyHostToDevice, st);
+}
+
+// ===================================================================================== emitter
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw 2011, "Parallel random numbers: as easy as 1, 2, 3").
+__device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint2 k)
+{
+    constexpr uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint32_t hi0 = __umulhi(M0, c.x), lo0 = M0 * c.x;
+        const uint32_t hi1 = __umulhi(M1, c.z), lo1 = M1 * c.z;
+        c = make_uint4(hi1 ^ c.y ^ k.x, lo1, hi0 ^ c.w ^ k.y, lo0);
+        k.x += W0;
+        k.y += W1;
+    }
+    return c;
+}
+
+// MPMEmitterKernel (MPM.cu:380-438): a width x rows lattice trailing the nozzle in the stream frame,
+// Gaussian position jitter (Box-Muller on two Philox outputs), v = velocity / tick, C = 0, F = I, J = 1.
+// Appended after count[1]; a wave64 ballot counts the lanes that fit the capacity into count[0].
+__global__ __launch_bounds__(kThreads) void k_emitter(DevParams p, Store s, int32_t* count, int cap, float ox,
+                                                      float oy, float velx, float vely, int width, float spacing,
+                                                      float jitter, int n_emit, uint64_t seed, float inv_tick)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = i < n_emit;
+    const float v_sqr = velx * velx + vely * vely;
+    const float inv_v = (v_sqr > 1e-12f) ? rsqrtf(v_sqr) : 0.f;
+    const float tx = velx * inv_v, ty = vely * inv_v;
+    const float nx = -ty, ny = tx;
+    float px = 0.f, py = 0.f;
+    int idx = 0;
+    if (active) {
+        const int col = i % width;
+        const int row = i / width;
+        const float across = ((float)col - 0.5f * (float)(width - 1)) * spacing;
+        const float along = -((float)row + 0.5f) * spacing;
+        px = ox + nx * across + tx * along;
+        py = oy + ny * across + ty * along;
+        if (jitter > 0.f) {
+            const uint4 r = philox4x32_10(make_uint4((uint32_t)i, 0u, 0u, 0u),
+                                          make_uint2((uint32_t)seed, (uint32_t)(seed >> 32)));
+            const float u1 = ((float)(r.x >> 8) + 1.f) * (1.f / 16777216.f);  // (0, 1]
+            const float u2 = (float)(r.y >> 8) * (1.f / 16777216.f);          // [0, 1)
+            const float rad = sqrtf(-2.f * logf(u1));
+            float sn, cs;
+            sincosf(6.2831853071795864f * u2, &sn, &cs);
+            px += rad * cs * jitter;
+            py += rad * sn * jitter;
+        }
+        idx = i + count[1];
+    }
+    const bool contributor = active && idx < cap;
+    const uint64_t mask = __ballot(contributor);
+    if (contributor) {
+        s.x[idx] = px;
+        s.y[idx] = py;
+        s.vx[idx] = velx * inv_tick;
+        s.vy[idx] = vely * inv_tick;
+        s.c00[idx] = 0.f;
+        s.c01[idx] = 0.f;
+        s.c10[idx] = 0.f;
+        s.c11[idx] = 0.f;
+        s.f00[idx] = 1.f;
+        s.f01[idx] = 0.f;
+        s.f10[idx] = 0.f;
+        s.f11[idx] = 1.f;
+        s.J[idx] = 1.f;
+        s.mat[idx] = MPM_MAT_FLUID;
+        s.id[idx] = idx;
+        s.key[idx] = 0u;
+    }
+    if ((threadIdx.x & 63) == 0 && mask != 0ull) { atomicAdd(&count[0], (int)__popcll(mask)); }
+}
+__global__ void k_commit_count(int32_t* count) { count[1] = count[0]; }
+
+hipError_t launch_emitter(hipStream_t st, const DevParams& p, const Store& s, int32_t* count, int cap, float ox,
+                          float oy, float velx, float vely, int width, float spacing, float jitter, int n_emit,
+                          uint64_t seed, float inv_tick)
+{
+    if (n_emit <= 0) { return hipSuccess; }
+    hipLaunchKernelGGL(k_emitter, dim3((unsigned)((n_emit + kThreads - 1) / kThreads)), dim3(kThreads), 0, st, p, s,
+                       count, cap, ox, oy, velx, vely, width, spacing, jitter, n_emit, seed, inv_tick);
+    hipLaunchKernelGGL(k_commit_count, dim3(1), dim3(1), 0, st, count);
+    return hipGetLastError();
+}
+
+// ===================================================================================== stress
+
+__global__ __launch_bounds__(kThreads) void k_stress(DevParams p, int n, const uint8_t* mat, const float* F,
+                                                     const float* J, float* tau)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) { return; }
+    const Mat2 f = {F[i], F[n + i], F[2 * n + i], F[3 * n + i]};
+    const int m = mat[i];
+    float t00, t01, t11;
+    if (m == MPM_MAT_FLUID) {
+        const float p_hat = p.tait_k * (powf(1.f / J[i], p.tait_gamma) - 1.f);
+        const float pr = fminf(fmaxf(p_hat, p.p_min), p.p_max);
+        t00 = -J[i] * pr;
+        t01 = 0.f;
+        t11 = -J[i] * pr;
+    } else {
+        float mu = p.mu, lambda = p.lambda;
+        if (m == MPM_MAT_SNOW) {
+            const float h = __expf(p.hardening * (1.f - J[i]));
+            mu *= h;
+            lambda *= h;
+        }
+        corotated_tau(f, mu, lambda, t00, t01, t11);
+    }
+    tau[i] = t00;
+    tau[n + i] = t01;
+    tau[2 * n + i] = t11;
+}
+
+hipError_t launch_stress(hipStream_t st, const DevParams& p, int n, const uint8_t* mat, const float* F, const float* J,
+                         float* tau)
+{
+    if (n <= 0) { return hipSuccess; }
+    hipLaunchKernelGGL(k_stress, dim3((unsigned)((n + kThreads - 1) / kThreads)), dim3(kThreads), 0, st, p, n, mat, F,
+                       J, tau);
+    return hipGetLastError();
+}
+
+}  // namespace mpm

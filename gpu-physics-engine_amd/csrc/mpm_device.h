@@ -1,0 +1,324 @@
+// mpm_device.h — device-side data layout and per-particle physics of the gfx950 MLS-MPM kernels.
+//
+// Everything here is __device__ inline and shared by the reference-structured, tiled and fused kernels,
+// so the three execution modes compute identical per-particle arithmetic and differ only in where the
+// grid values live (HBM or an LDS tile) and in the order of the floating-point atomic sums.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/mpm_engine.h"
+
+namespace mpm {
+
+// ---- launch constants ----
+constexpr int kThreads = 256;        // work-group size of every particle / node kernel (4 wave64)
+constexpr int kRadixBits = 8;        // digit width of the binning sort
+constexpr int kRadixBins = 1 << kRadixBits;
+constexpr int kRadixItems = 16;      // keys per thread per sort tile
+constexpr int kRadixTile = kThreads * kRadixItems;
+constexpr int kMargin = 2;           // nodes of drift a binned particle may take before leaving its tile
+constexpr int kMaxScenes = 4096;
+
+// Kernel-argument copy of MpmParams plus derived quantities (passed by value, lives in SGPRs/kernarg).
+struct DevParams {
+    int W, H, halo, wall;
+    int G;              // nodes per scene
+    int num_scenes;
+    int nbx, nby, bps;  // bins per scene along x, y and in total
+    int bin_x, bin_y;   // cells per bin
+    float dx, inv_dx, sim_w, sim_h, dt;
+    float gdt_x, gdt_y;  // gravity * dt, folded once like the reference's constexpr kGravity * dt (MPM.cu:154)
+    float mass, vol;
+    float tait_k, tait_gamma, p_min, p_max;
+    float mu, lambda, theta_c, theta_s, hardening;
+    float j_min, j_max;
+    int closure;
+};
+
+// SoA particle store.  Every component array holds `capacity` entries, 256-byte aligned.
+struct Store {
+    float* x;
+    float* y;
+    float* vx;
+    float* vy;
+    float* c00;
+    float* c01;
+    float* c10;
+    float* c11;
+    float* f00;
+    float* f01;
+    float* f10;
+    float* f11;
+    float* J;       // tracked J (fluid, jelly) or plastic Jp (snow)
+    uint8_t* mat;
+    int32_t* id;    // caller's particle index
+    uint32_t* key;  // bin key of the last binning (scene-major), carries the scene of the particle
+};
+
+// Grid of all scenes: scene s, node (x, y) at s*G + y*W + x (reference node index, MPM.cu:122).
+struct Grid {
+    float* mom_x;   // P2G accumulators, zero between substeps
+    float* mom_y;
+    float* mass;
+    float2* vel;    // grid-update output read by G2P
+    float* vmass;   // mass of the last grid update (readback)
+};
+
+// Work item of the binned kernels: particles [begin, end) of bin `key` (all within one bin).
+struct Item {
+    int32_t key;
+    int32_t begin;
+    int32_t end;
+    int32_t pad;
+};
+
+struct Mat2 {
+    float a00, a01, a10, a11;
+};
+
+// ---- B-spline weights (MPM.cu:92-110) ----
+// The reference's gridCoord x * inv_dx + halo (MPM.cu:84) is contracted to one fused multiply-add by
+// nvcc; it is written as an explicit fmaf here so the base node (and the bin key) is reproducible.
+__device__ __forceinline__ void bspline(float x, float inv_dx, float halo, int& base, float& f, float w[3])
+{
+    const float g = fmaf(x, inv_dx, halo);
+    base = (int)floorf(g - 0.5f);
+    f = g - (float)base;
+    const float a = 1.5f - f;
+    const float b = f - 1.0f;
+    const float c = f - 0.5f;
+    w[0] = 0.5f * a * a;
+    w[1] = 0.75f - b * b;
+    w[2] = 0.5f * c * c;
+}
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// ---- 2x2 helpers (det2D / matMul2D, cuda-internal.cuh:44-56) ----
+__device__ __forceinline__ float det2(const Mat2& m) { return m.a00 * m.a11 - m.a01 * m.a10; }
+__device__ __forceinline__ Mat2 mul2(const Mat2& A, const Mat2& B)
+{
+    return {A.a00 * B.a00 + A.a01 * B.a10, A.a00 * B.a01 + A.a01 * B.a11, A.a10 * B.a00 + A.a11 * B.a10,
+            A.a10 * B.a01 + A.a11 * B.a11};
+}
+
+// Rotation factor of the polar decomposition F = R S (the rotation maximising tr(R^T F)):
+// angle atan2(F10 - F01, F00 + F11), R = I when both vanish.  Returns cos, sin.
+__device__ __forceinline__ void polar_cs_f32(const Mat2& F, float& c, float& s)
+{
+    const float x = F.a00 + F.a11;
+    const float y = F.a10 - F.a01;
+    const float r2 = x * x + y * y;
+    if (r2 > 0.f) {
+        const float inv = rsqrtf(r2);
+        c = x * inv;
+        s = y * inv;
+    } else {
+        c = 1.f;
+        s = 0.f;
+    }
+}
+// Same in FP64, used only for near-degenerate F (det F < 0.1 or det F > 10).
+__device__ __noinline__ void polar_cs_f64(const Mat2& F, float& c, float& s)
+{
+    const double x = (double)F.a00 + (double)F.a11;
+    const double y = (double)F.a10 - (double)F.a01;
+    const double r = sqrt(x * x + y * y);
+    if (r > 0.0) {
+        c = (float)(x / r);
+        s = (float)(y / r);
+    } else {
+        c = 1.f;
+        s = 0.f;
+    }
+}
+__device__ __forceinline__ void polar_cs(const Mat2& F, float& c, float& s)
+{
+    const float J = det2(F);
+    if (J < 0.1f || J > 10.f) {
+        polar_cs_f64(F, c, s);
+    } else {
+        polar_cs_f32(F, c, s);
+    }
+}
+
+// Fixed-corotated Kirchhoff stress tau = 2 mu (F - R) F^T + lambda (J - 1) J I, J = det F
+// (Stomakhin et al. 2012).  (F - R) F^T = F F^T - R F^T and R F^T = R S R^T is symmetric, so tau is
+// symmetric; returns tau00, tau01 (= tau10), tau11.
+__device__ __forceinline__ void corotated_tau(const Mat2& F, float mu, float lambda, float& t00, float& t01,
+                                              float& t11)
+{
+    float c, s;
+    polar_cs(F, c, s);
+    const float J = det2(F);
+    // D = F - R, R = [[c, -s], [s, c]]
+    const float d00 = F.a00 - c, d01 = F.a01 + s, d10 = F.a10 - s, d11 = F.a11 - c;
+    const float m00 = d00 * F.a00 + d01 * F.a01;
+    const float m01 = d00 * F.a10 + d01 * F.a11;
+    const float m11 = d10 * F.a10 + d11 * F.a11;
+    const float vol = lambda * (J - 1.f) * J;
+    const float two_mu = 2.f * mu;
+    t00 = two_mu * m00 + vol;
+    t01 = two_mu * m01;
+    t11 = two_mu * m11 + vol;
+}
+
+// Signed SVD F = U diag(s0, s1) V^T with U, V rotations: polar F = R S, then one Jacobi rotation
+// diagonalising the symmetric S.  Outputs cos/sin of U and V.
+__device__ __forceinline__ void svd2(const Mat2& F, float& uc, float& us, float& s0, float& s1, float& vc,
+                                     float& vs)
+{
+    float c, s;
+    polar_cs(F, c, s);
+    // S = R^T F, R = [[c, -s], [s, c]]
+    const float S00 = c * F.a00 + s * F.a10;
+    const float S01 = c * F.a01 + s * F.a11;
+    const float S10 = -s * F.a00 + c * F.a10;
+    const float S11 = -s * F.a01 + c * F.a11;
+    const float off = 0.5f * (S01 + S10);
+    float jc = 1.f, js = 0.f;
+    if (off != 0.f) {
+        const float tau = 0.5f * (S11 - S00) / off;
+        const float t = copysignf(1.f, tau) / (fabsf(tau) + sqrtf(1.f + tau * tau));
+        jc = rsqrtf(1.f + t * t);
+        js = t * jc;
+    }
+    // V = [[jc, js], [-js, jc]]
+    s0 = jc * jc * S00 - 2.f * jc * js * off + js * js * S11;
+    s1 = js * js * S00 + 2.f * jc * js * off + jc * jc * S11;
+    vc = jc;
+    vs = -js;  // V as a rotation: [[vc, -vs], [vs, vc]] = [[jc, js], [-js, jc]]
+    // U = R V: rotation by angle(R) + angle(V)
+    uc = c * vc - s * vs;
+    us = s * vc + c * vs;
+}
+
+// Snow return mapping (Stomakhin et al. 2013): clamp the singular values of the trial elastic F to
+// [1 - theta_c, 1 + theta_s], move the removed volume into Jp, rebuild F = U diag(s') V^T.
+__device__ __forceinline__ void snow_project(const DevParams& p, Mat2& F, float& Jp)
+{
+    float uc, us, s0, s1, vc, vs;
+    svd2(F, uc, us, s0, s1, vc, vs);
+    const float lo = 1.f - p.theta_c, hi = 1.f + p.theta_s;
+    const float n0 = fminf(fmaxf(s0, lo), hi);
+    const float n1 = fminf(fmaxf(s1, lo), hi);
+    Jp *= (s0 * s1) / (n0 * n1);
+    // U = [[uc, -us], [us, uc]], V = [[vc, -vs], [vs, vc]]; F = U diag(n) V^T
+    const float u00 = uc, u01 = -us, u10 = us, u11 = uc;
+    const float v00 = vc, v01 = -vs, v10 = vs, v11 = vc;
+    F.a00 = u00 * n0 * v00 + u01 * n1 * v01;
+    F.a01 = u00 * n0 * v10 + u01 * n1 * v11;
+    F.a10 = u10 * n0 * v00 + u11 * n1 * v01;
+    F.a11 = u10 * n0 * v10 + u11 * n1 * v11;
+}
+
+// Affine momentum matrix Q of P2G.  Fluid follows the reference expression order exactly
+// (MPM.cu:68-80: Tait pressure on J, V = V0 J, coeff = -dt V 4 inv_dx^2, Q = m C + coeff sigma);
+// jelly and snow use Q = m C - dt V0 4 inv_dx^2 tau with tau the fixed-corotated Kirchhoff stress.
+__device__ __forceinline__ Mat2 affine_q(const DevParams& p, int mat, const Mat2& C, const Mat2& F, float J)
+{
+    if (mat == MPM_MAT_FLUID) {
+        const float p_hat = p.tait_k * (powf(1.f / J, p.tait_gamma) - 1.f);
+        const float pressure = fminf(fmaxf(p_hat, p.p_min), p.p_max);
+        const float V = p.vol * J;
+        const float coeff = -p.dt * V * 4.f * p.inv_dx * p.inv_dx;
+        const float sig = -pressure;
+        return {p.mass * C.a00 + coeff * sig, p.mass * C.a01 + coeff * 0.f, p.mass * C.a10 + coeff * 0.f,
+                p.mass * C.a11 + coeff * sig};
+    }
+    float mu = p.mu, lambda = p.lambda;
+    if (mat == MPM_MAT_SNOW) {
+        const float h = __expf(p.hardening * (1.f - J));
+        mu *= h;
+        lambda *= h;
+    }
+    float t00, t01, t11;
+    corotated_tau(F, mu, lambda, t00, t01, t11);
+    const float D = p.dt * p.vol * 4.f * p.inv_dx * p.inv_dx;
+    return {p.mass * C.a00 - D * t00, p.mass * C.a01 - D * t01, p.mass * C.a10 - D * t01, p.mass * C.a11 - D * t11};
+}
+
+// handleOOB (MPM.cu:267-301).
+__device__ __forceinline__ void handle_oob(const DevParams& p, float& px, float& py, float& vx, float& vy)
+{
+    if (!isfinite(px) || !isfinite(vx)) {
+        px = 0.5f * p.sim_w;
+        vx = 0.f;
+    }
+    if (!isfinite(py) || !isfinite(vy)) {
+        py = 0.5f * p.sim_h;
+        vy = 0.f;
+    }
+    if (px < 0.f) {
+        px = 0.f;
+        vx = fmaxf(vx, 0.f);
+    }
+    if (px > p.sim_w) {
+        px = p.sim_w;
+        vx = fminf(vx, 0.f);
+    }
+    if (py < 0.f) {
+        py = 0.f;
+        vy = fmaxf(vy, 0.f);
+    }
+    if (py > p.sim_h) {
+        py = p.sim_h;
+        vy = fminf(vy, 0.f);
+    }
+}
+
+// Particle update after the gather (MPM.cu:252-265): advect, C = 4 inv_dx^2 B (before the OOB handling),
+// out-of-bounds, F <- (I + dt C) F, then J <- clamp(J det A) or, for snow, the return mapping on F and Jp.
+__device__ __forceinline__ void g2p_update(const DevParams& p, int mat, float& px, float& py, float& vx, float& vy,
+                                           const Mat2& B, Mat2& C, Mat2& F, float& J)
+{
+    const float sB = 4.f * p.inv_dx * p.inv_dx;
+    px += vx * p.dt;
+    py += vy * p.dt;
+    C = {sB * B.a00, sB * B.a01, sB * B.a10, sB * B.a11};
+    handle_oob(p, px, py, vx, vy);
+    const Mat2 A = {1.f + C.a00 * p.dt, C.a01 * p.dt, C.a10 * p.dt, 1.f + C.a11 * p.dt};
+    F = mul2(A, F);
+    if (mat == MPM_MAT_SNOW) {
+        snow_project(p, F, J);
+    } else {
+        J *= det2(A);
+        J = fminf(fmaxf(J, p.j_min), p.j_max);
+    }
+}
+
+// Grid node update (MPM.cu:150-172): momentum / mass, gravity, outward-only walls, empty node -> 0.
+__device__ __forceinline__ float2 node_velocity(const DevParams& p, float mx, float my, float m, int x, int y)
+{
+    float2 v = {0.f, 0.f};
+    if (m > 0.f) {
+        const float inv_m = 1.f / m;
+        v.x = mx * inv_m + p.gdt_x;
+        v.y = my * inv_m + p.gdt_y;
+        if (x <= p.wall && v.x < 0.f) {
+            v.x = 0.f;
+        } else if (x >= p.W - 1 - p.wall && v.x > 0.f) {
+            v.x = 0.f;
+        }
+        if (y <= p.wall && v.y < 0.f) {
+            v.y = 0.f;
+        } else if (y >= p.H - 1 - p.wall && v.y > 0.f) {
+            v.y = 0.f;
+        }
+    }
+    return v;
+}
+
+// Bin key of a position (scene-major); base node clamped for memory safety.
+__device__ __forceinline__ uint32_t bin_key(const DevParams& p, float x, float y, uint32_t scene)
+{
+    int bx = (int)floorf(fmaf(x, p.inv_dx, (float)p.halo) - 0.5f);
+    int by = (int)floorf(fmaf(y, p.inv_dx, (float)p.halo) - 0.5f);
+    bx = clampi(bx, 0, p.W - 3);
+    by = clampi(by, 0, p.H - 3);
+    return scene * (uint32_t)p.bps + (uint32_t)((by / p.bin_y) * p.nbx + bx / p.bin_x);
+}
+
+}  // namespace mpm

@@ -1,0 +1,182 @@
+"""Python mirror of the reference engine surface for the MPM path, over the C-ABI of include/mpm_engine.h.
+
+Reference interface (libs/cuda-engine/include/cuda-engine/cuda-engine.h) and its counterpart here:
+  MPMArena() + instantiateMPMGraph()        -> Engine(params)            (mpm_create)
+  hosePipe(stream, mpm, ...)                -> Engine.hose_pipe(...)      (mpm_hose_pipe)
+  cudaGraphLaunch per tick                  -> Engine.step(n_ticks)       (mpm_step)
+  cudaEventQuery(tick_done)                 -> Engine.query()             (mpm_query)
+  cudaEventElapsedTime(t0, t1)              -> Engine.last_step_ms()      (mpm_last_step_ms)
+  OutputBuffer::readTarget()                -> Engine.read_view()         (mpm_read_view)
+plus host readbacks for tests (Engine.readback / Engine.read_grid).  Every call raises MpmError on a
+negative status; there is no fallback path.
+"""
+import ctypes
+
+import numpy as np
+
+from . import abi
+
+
+class MpmError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__(f"{abi.load_library().mpm_error_name(code).decode()} ({code}): {message}")
+        self.code = code
+
+
+def _check(lib, h, rc, what):
+    if rc < 0:
+        msg = lib.mpm_last_message(h)
+        raise MpmError(rc, f"{what}: {msg.decode() if msg else ''}")
+    return rc
+
+
+def _fp(a):
+    return None if a is None else a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+class Engine:
+    """One engine handle: SoA particle store, grids, tick graphs and the output triple buffer."""
+
+    def __init__(self, params):
+        self.lib = abi.load_library()
+        self.params = params.copy()
+        h = ctypes.c_void_p()
+        rc = self.lib.mpm_create(ctypes.byref(self.params), ctypes.byref(h))
+        if rc < 0:
+            msg = self.lib.mpm_last_message(None)
+            raise MpmError(rc, f"mpm_create: {msg.decode() if msg else ''}")
+        self.h = h
+        self.n = 0
+
+    # -- lifetime --------------------------------------------------------------------------------------
+    def close(self):
+        if self.h is not None and self.h.value:
+            self.lib.mpm_destroy(self.h)
+        self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- population ------------------------------------------------------------------------------------
+    def set_particles(self, x, v=None, C=None, F=None, J=None, material=None, scene_counts=None):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 2)
+        n = x.shape[0]
+
+        def arr(a, cols, dt=np.float32):
+            if a is None:
+                return None
+            return np.ascontiguousarray(a, dtype=dt).reshape(n, cols) if cols > 1 else \
+                np.ascontiguousarray(a, dtype=dt).reshape(n)
+        v, C, F, J = arr(v, 2), arr(C, 4), arr(F, 4), arr(J, 1)
+        m = arr(material, 1, np.uint8)
+        sc = None if scene_counts is None else np.ascontiguousarray(scene_counts, dtype=np.int32)
+        rc = self.lib.mpm_set_particles(
+            self.h, n, _fp(x), _fp(v), _fp(C), _fp(F), _fp(J),
+            None if m is None else m.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+            None if sc is None else sc.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+        _check(self.lib, self.h, rc, "mpm_set_particles")
+        self.n = n
+
+    def set_scene(self, scene):
+        self.set_particles(scene.x, scene.v, material=scene.material,
+                           scene_counts=scene.scene_counts if self.params.num_scenes > 1 else None)
+
+    def hose_pipe(self, origin, velocity, nozzle_width, spacing, jitter, tick):
+        out = ctypes.c_int32()
+        rc = self.lib.mpm_hose_pipe(self.h, float(origin[0]), float(origin[1]), float(velocity[0]),
+                                    float(velocity[1]), float(nozzle_width), float(spacing), float(jitter),
+                                    int(tick), ctypes.byref(out))
+        _check(self.lib, self.h, rc, "mpm_hose_pipe")
+        return out.value
+
+    # -- stepping --------------------------------------------------------------------------------------
+    def step(self, n_ticks=1):
+        _check(self.lib, self.h, self.lib.mpm_step(self.h, int(n_ticks)), "mpm_step")
+
+    def query(self):
+        return _check(self.lib, self.h, self.lib.mpm_query(self.h), "mpm_query") == abi.MPM_OK
+
+    def sync(self):
+        _check(self.lib, self.h, self.lib.mpm_sync(self.h), "mpm_sync")
+
+    def last_step_ms(self):
+        ms = ctypes.c_float()
+        _check(self.lib, self.h, self.lib.mpm_last_step_ms(self.h, ctypes.byref(ms)), "mpm_last_step_ms")
+        return ms.value
+
+    # -- readback --------------------------------------------------------------------------------------
+    def count(self):
+        n = ctypes.c_int32()
+        _check(self.lib, self.h, self.lib.mpm_count(self.h, ctypes.byref(n)), "mpm_count")
+        return n.value
+
+    def readback(self):
+        n = self.count()
+        x = np.zeros((n, 2), np.float32)
+        v = np.zeros((n, 2), np.float32)
+        C = np.zeros((n, 4), np.float32)
+        F = np.zeros((n, 4), np.float32)
+        J = np.zeros(n, np.float32)
+        m = np.zeros(n, np.uint8)
+        got = ctypes.c_int32()
+        rc = self.lib.mpm_readback(self.h, _fp(x), _fp(v), _fp(C), _fp(F), _fp(J),
+                                   m.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), ctypes.byref(got))
+        _check(self.lib, self.h, rc, "mpm_readback")
+        return {"x": x, "v": v, "C": C, "F": F, "J": J, "material": m}
+
+    def read_grid(self):
+        p = self.params
+        nodes = p.num_scenes * p.grid_w * p.grid_h
+        mass = np.zeros(nodes, np.float32)
+        vx = np.zeros(nodes, np.float32)
+        vy = np.zeros(nodes, np.float32)
+        _check(self.lib, self.h, self.lib.mpm_read_grid(self.h, _fp(mass), _fp(vx), _fp(vy)), "mpm_read_grid")
+        shape = (p.num_scenes, p.grid_h, p.grid_w)
+        return mass.reshape(shape), vx.reshape(shape), vy.reshape(shape)
+
+    def read_view(self):
+        view = abi.MpmView()
+        _check(self.lib, self.h, self.lib.mpm_read_view(self.h, ctypes.byref(view)), "mpm_read_view")
+        return view
+
+    # -- diagnostics (include/mpm_diag.h) --------------------------------------------------------------
+    def debug_bin(self):
+        """Bin the current state; returns (keys, ids) in binned order."""
+        n = self.count()
+        keys = np.zeros(n, np.uint32)
+        ids = np.zeros(n, np.int32)
+        got = ctypes.c_int32()
+        rc = self.lib.mpm_debug_bin(self.h, keys.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                    ids.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), ctypes.byref(got))
+        _check(self.lib, self.h, rc, "mpm_debug_bin")
+        return keys, ids
+
+    def time_kernels(self, substeps):
+        """Advance `substeps` eagerly with HIP events around every launch; returns {class: (ms, launches)}."""
+        ms = np.zeros(abi.KT_COUNT, np.float32)
+        cnt = np.zeros(abi.KT_COUNT, np.int32)
+        rc = self.lib.mpm_time_kernels(self.h, int(substeps), _fp(ms),
+                                       cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+        _check(self.lib, self.h, rc, "mpm_time_kernels")
+        return {abi.KT_NAMES[k]: (float(ms[k]), int(cnt[k])) for k in range(abi.KT_COUNT)}
+
+    def compute_stress(self, material, F, J):
+        """Kirchhoff stress (tau00, tau01, tau11) per particle from the engine's device code."""
+        F = np.ascontiguousarray(F, dtype=np.float32).reshape(-1, 4)
+        n = F.shape[0]
+        J = np.ascontiguousarray(J, dtype=np.float32).reshape(n)
+        m = np.ascontiguousarray(material, dtype=np.uint8).reshape(n)
+        tau = np.zeros((n, 3), np.float32)
+        rc = self.lib.mpm_compute_stress(self.h, n, m.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), _fp(F), _fp(J),
+                                         _fp(tau))
+        _check(self.lib, self.h, rc, "mpm_compute_stress")
+        return tau

@@ -1,0 +1,173 @@
+/*
+ * mpm_engine.h — C-ABI of the MI355X MLS-MPM engine.
+ *
+ * This header replaces the C++ surface of the reference engine
+ * (libs/cuda-engine/include/cuda-engine/cuda-engine.h) for the MPM path:
+ *   - MPMArena         (cuda-engine.h:271-297, cuda-engine.cpp:234-263)  -> mpm_create / mpm_set_particles
+ *   - GraphEnv         (cuda-engine.h:97-109)                            -> owned by the handle
+ *   - instantiateMPMGraph (cuda-engine.h:308-311, MPM.cu:326-374)        -> mpm_create (graph built inside)
+ *   - cudaGraphLaunch + t0/t1/tick_done events (main/MPM-master.cpp:368-371)
+ *                                                                        -> mpm_step / mpm_query / mpm_sync /
+ *                                                                           mpm_last_step_ms
+ *   - OutputBuffer / View / MPMTickStaging (cuda-engine.h:132-161, 312-342; MPM.cu:19-44, 302-322)
+ *                                                                        -> mpm_read_view / mpm_roll_output
+ *   - hosePipe         (cuda-engine.h:298-307, MPM.cu:439-473)            -> mpm_hose_pipe
+ * and follows the error convention of the reference's only C-ABI (libs/render/include/render/cuda-interface.h:30-42,
+ * libs/render/source/cuda-render.cu:23-27): int32_t status, 0 = ok, negative = error, plus last_error /
+ * error_name queries.  No exception or abort crosses this boundary.
+ *
+ * All pointers are plain host pointers unless a comment says "device pointer".  Vector fields use the
+ * reference's interleaved layout at the boundary: positions and velocities as float[2*n] (x0,y0,x1,y1,...)
+ * like float2[], and C / F as float[4*n] row-major (F00,F01,F10,F11) like the reference's float4
+ * (cuda-engine.h:280-285, cuda-internal.cuh:44-56).  Inside the engine the store is SoA per scalar.
+ *
+ * Threading: one host thread drives a handle (like the reference's sim thread).  mpm_read_view may be
+ * called from a second thread (the reference's renderer thread reads OutputBuffer::readTarget,
+ * cuda-engine.cpp:166-175).  Several handles may coexist; batched scenes live inside one handle.
+ */
+#ifndef MPM_ENGINE_H
+#define MPM_ENGINE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (reference render C-ABI uses 0 / -1 / -2 / -4 / -5, cuda-render.cu:23-60) ---- */
+#define MPM_OK 0
+#define MPM_BUSY 1                  /* mpm_query: work still in flight (cudaErrorNotReady, MPM-master.cpp:345) */
+#define MPM_ERR_INVALID_ARG (-1)    /* null handle/pointer, bad size, parameter out of range               */
+#define MPM_ERR_DEVICE (-2)         /* a HIP runtime call failed; mpm_last_error holds the hipError_t       */
+#define MPM_ERR_CAPACITY (-3)       /* particle count above the handle's capacity                            */
+#define MPM_ERR_NO_DEVICE (-4)      /* no usable gfx950 device                                               */
+#define MPM_ERR_STATE (-5)          /* call not valid in the handle's current state                          */
+#define MPM_ERR_NOT_BUILT (-6)      /* requested code path is not compiled into this library                 */
+
+/* ---- materials (per particle, uint8) ---- */
+#define MPM_MAT_FLUID 0 /* Tait equation of state on the tracked J (reference MPM.cu:68-80)             */
+#define MPM_MAT_JELLY 1 /* fixed-corotated elasticity on F (Stomakhin et al. 2012)                      */
+#define MPM_MAT_SNOW 2  /* fixed-corotated + singular-value clamp plasticity (Stomakhin et al. 2013)   */
+
+/* ---- execution modes ---- */
+#define MPM_MODE_REFERENCE 0 /* reference kernel structure: per-particle P2G with HBM atomics, grid, G2P   */
+#define MPM_MODE_TILED 1     /* binned particles, LDS-tiled P2G, grid, LDS-tiled G2P (three phases)         */
+#define MPM_MODE_FUSED 2     /* binned particles, grid update + fused G2P->P2G (two phases per substep)     */
+
+/* ---- constitutive closure ---- */
+#define MPM_CLOSURE_ANALYTIC 0 /* polar decomposition + fixed-corotated / snow stress                     */
+#define MPM_CLOSURE_LEARNED 1  /* BF16 MLP on (sigma1, sigma2, angle, J), analytic outside trained range */
+
+/* ---- named parameter presets ---- */
+#define MPM_PRESET_REFERENCE 0      /* reference constants.h: 196x100 nodes, dx=2, Tait fluid, 20 substeps */
+#define MPM_PRESET_JELLY128 1       /* 128-cell unit square, dx=1/128, jelly E=1e3, dt=1e-4              */
+#define MPM_PRESET_MIXED1024 2      /* 1024-cell unit square, fluid/jelly/snow boxes, dt=2e-5            */
+#define MPM_PRESET_BATCH64X256 3    /* 64 scenes of 256 cells, mixed materials                           */
+#define MPM_PRESET_LEARNED1024 4    /* MIXED1024 grid, jelly + snow, learned closure                     */
+#define MPM_PRESET_LEARNED16X2048 5 /* 16 scenes of 2048 cells, jelly + snow, learned closure           */
+
+/*
+ * Simulation parameters.  Every compile-time constant of the reference (constants.h:9-88) is a field
+ * here; mpm_params_preset(MPM_PRESET_REFERENCE) reproduces the reference values.
+ */
+typedef struct MpmParams {
+    /* grid: nodes per scene including the halo (reference kGridDimX/Y = sim/dx + 2*kGridHalo, constants.h:76-77) */
+    int32_t grid_w, grid_h;
+    int32_t halo;      /* node offset of the domain origin, >= 2 (kGridHalo, constants.h:74)                  */
+    int32_t wall;      /* node index <= wall or >= dim-1-wall zeroes the outward velocity (MPM.cu:159-168)   */
+    float dx, inv_dx;  /* cell size and its reciprocal (kDx, inv_dx, constants.h:80-81)                       */
+    float sim_w, sim_h;/* domain extent for the out-of-bounds clamp (kSimWidth/Height, MPM.cu:336-337)       */
+    float dt;          /* substep (constants.h:88)                                                            */
+    float gravity_x, gravity_y; /* added to node velocity times dt (kGravity on y, MPM.cu:154)              */
+    float mass, vol;   /* particle mass and rest volume (kMass, kRestVol)                                     */
+    float tait_k, tait_gamma, p_min, p_max; /* fluid: p = k((1/J)^gamma - 1) clamped (constants.h:84-87)     */
+    float mu, lambda;  /* Lame parameters of jelly and snow                                                   */
+    float theta_c, theta_s, hardening; /* snow: clamp [1-theta_c, 1+theta_s], mu,lambda *= exp(h(1-Jp))    */
+    float j_min, j_max;/* clamp of the tracked J (MPM.cu:265)                                                 */
+    int32_t substeps;  /* substeps per mpm_step tick (kSubsteps, constants.h:72)                              */
+    int32_t capacity;  /* particle capacity over all scenes (kNumAtoms, constants.h:83)                       */
+    int32_t num_scenes;/* independent scenes, each with its own grid_w x grid_h grid (batched form)           */
+    int32_t sort_interval; /* substeps between re-binnings inside a tick (>=1; binned modes only)            */
+    int32_t mode;      /* MPM_MODE_*                                                                          */
+    int32_t closure;   /* MPM_CLOSURE_*                                                                       */
+    int32_t block_x, block_y; /* cells per bin along x / y (0 = tuned default)                               */
+    int32_t chunk;     /* particles per work item in the binned kernels (0 = tuned default)                   */
+    int32_t out_chunk; /* particles per output chunk for View.num_atoms (kMaxAtoms = 1536, constants.h:50)   */
+    int32_t reserved[7];
+} MpmParams;
+
+/* Device pointers of one output slot (reference View, cuda-engine.h:132-138). */
+typedef struct MpmView {
+    float* pos;         /* device pointer, float[2*capacity], particle i at pos[2i], pos[2i+1]             */
+    float* vel;         /* device pointer, float[2*capacity]                                             */
+    int32_t* num_atoms; /* device pointer, int32[ceil(capacity/out_chunk)] particles per output chunk      */
+    int32_t slot;       /* 0..2, -1 before the first tick                                                  */
+    int32_t count;      /* particles written to this slot                                                  */
+} MpmView;
+
+typedef struct MpmEngine MpmEngine;
+
+/* Fill *out with a named preset (MPM_PRESET_*).  Replaces the constexpr block of constants.h:9-88. */
+int32_t mpm_params_preset(int32_t preset, MpmParams* out);
+
+/* Allocate the SoA store, grids and output triple buffer on the current device, and build the tick
+ * hipGraph.  Replaces MPMArena() (cuda-engine.cpp:234-259), OutputBuffer (cuda-engine.h:143-161),
+ * instantiateMPMGraph (MPM.cu:326-374) and MPMTickStaging::init (MPM.cu:19-30). */
+int32_t mpm_create(const MpmParams* params, MpmEngine** out);
+/* Release everything the handle owns (~MPMArena, ~GraphEnv, ~OutputArena: cuda-engine.cpp:121-132, 151-154, 260-263). */
+int32_t mpm_destroy(MpmEngine* h);
+
+/*
+ * Replace the particle population (host arrays).  x, v: float[2n]; C, F: float[4n]; J: float[n];
+ * material: uint8[n].  v, C, F, J, material may be NULL: v = 0, C = 0, F = I, J = 1 (the emitter's
+ * initial state, MPM.cu:427-431), material = MPM_MAT_FLUID.  scene_counts: int32[num_scenes] particles
+ * per scene, particles grouped by scene in order (NULL allowed when num_scenes == 1).  Particle i keeps
+ * index i in every readback regardless of the engine's internal (binned) order.
+ */
+int32_t mpm_set_particles(MpmEngine* h, int32_t n, const float* x, const float* v, const float* C,
+                          const float* F, const float* J, const uint8_t* material, const int32_t* scene_counts);
+
+/* Enqueue n_ticks ticks of params.substeps substeps each, asynchronously (one graph launch per tick,
+ * MPM-master.cpp:368-371).  Each tick ends with the output copy into the next output slot. */
+int32_t mpm_step(MpmEngine* h, int32_t n_ticks);
+/* MPM_OK when all enqueued work finished, MPM_BUSY while in flight (cudaEventQuery, MPM-master.cpp:345). */
+int32_t mpm_query(MpmEngine* h);
+/* Block until all enqueued work finished. */
+int32_t mpm_sync(MpmEngine* h);
+/* GPU time of the last tick in ms (cudaEventElapsedTime(t0, t1), MPM-master.cpp:352-354). */
+int32_t mpm_last_step_ms(MpmEngine* h, float* ms);
+
+/* Device pointers of the slot most recently completed by a tick (OutputBuffer::readTarget, cuda-engine.cpp:166-175).
+ * Safe to call from a second thread. */
+int32_t mpm_read_view(MpmEngine* h, MpmView* out);
+
+/* Synchronous host copy of the particle state in the caller's particle order.  Any pointer may be NULL.
+ * Sizes as in mpm_set_particles; *n receives the current count. */
+int32_t mpm_readback(MpmEngine* h, float* x, float* v, float* C, float* F, float* J, uint8_t* material,
+                     int32_t* n);
+/* Synchronous host copy of the grid of the last substep: mass and velocity per node, float[num_scenes*grid_w*grid_h]
+ * each, node (scene, y, x) at scene*grid_w*grid_h + y*grid_w + x (reference node index, MPM.cu:122). */
+int32_t mpm_read_grid(MpmEngine* h, float* mass, float* vel_x, float* vel_y);
+/* Current particle count (device-resident count slot 0, cuda-engine.h:287-289). */
+int32_t mpm_count(MpmEngine* h, int32_t* n);
+
+/* Hose-pipe emitter (hosePipe, MPM.cu:439-473): enqueue width*rows particles on the handle's stream in
+ * scene 0.  origin/velocity in scene units, velocity is the per-tick displacement (divided by the
+ * tick length substeps*dt); jitter is the standard deviation of the Gaussian position jitter
+ * (Philox4x32-10 + Box-Muller, counter = particle index, key = tick).  *emitted receives the count
+ * enqueued; particles past capacity are dropped on the device like MPM.cu:422-436. */
+int32_t mpm_hose_pipe(MpmEngine* h, float origin_x, float origin_y, float velocity_x, float velocity_y,
+                      float nozzle_width, float spacing, float jitter, uint64_t tick, int32_t* emitted);
+
+/* Last hipError_t (or MPM_ERR_*) recorded on the handle (cuda_engine_last_error, cuda-interface.h:41);
+ * h may be NULL for errors raised before a handle existed. */
+int32_t mpm_last_error(MpmEngine* h);
+/* Static name of a status code (cuda_engine_error_name, cuda-interface.h:42). */
+const char* mpm_error_name(int32_t err);
+/* Human readable message of the last error on the handle (NULL handle: last create failure). */
+const char* mpm_last_message(MpmEngine* h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MPM_ENGINE_H */

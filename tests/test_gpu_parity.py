@@ -1,0 +1,291 @@
+"""GPU parity of the HIP engine against the FP64 oracle, through the C-ABI.
+
+Tolerances (declared): the only source of run-to-run non-determinism on the GPU is the order of the
+floating-point atomic sums in P2G; FP32 rounding plus that order noise is bounded here by
+  - positions: relative L2 error ||x_gpu - x_oracle|| / ||x_oracle||  (per test, 1e-6 .. 1e-4),
+  - velocities, C, F, J, grid mass: relative L2 error with the tolerance written in each test.
+"""
+import numpy as np
+import pytest
+
+from mpm_amd import abi, scenes
+from mpm_amd.engine import Engine
+
+pytestmark = pytest.mark.gpu
+
+MODES = [abi.MODE_REFERENCE, abi.MODE_TILED, abi.MODE_FUSED]
+MODE_NAMES = {abi.MODE_REFERENCE: "reference", abi.MODE_TILED: "tiled", abi.MODE_FUSED: "fused"}
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    den = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (den if den > 0 else 1.0)
+
+
+def run_gpu(params, x, v=None, material=None, ticks=1, scene_counts=None, C=None, F=None, J=None):
+    with Engine(params) as e:
+        e.set_particles(x, v, C=C, F=F, J=J, material=material, scene_counts=scene_counts)
+        e.step(ticks)
+        e.sync()
+        out = e.readback()
+        out["grid"] = e.read_grid()
+    return out
+
+
+def run_oracle(oracle_mod, params, x, v, material, substeps, threads=8, C=None, F=None, J=None):
+    s = oracle_mod.State(np.asarray(x, np.float64), None if v is None else np.asarray(v, np.float64),
+                         C=C, F=F, J=J, material=material)
+    grid = oracle_mod.run(params, s, substeps, threads=threads, want_grid=True)
+    return s, grid
+
+
+def small_params(cells=64, substeps=1, mode=abi.MODE_FUSED, dt=1e-4, rho=1.0, capacity=None, block=(16, 16)):
+    p = abi.preset(abi.PRESET_JELLY128)
+    p.grid_w = p.grid_h = cells + 4
+    p.dx = 1.0 / cells
+    p.inv_dx = float(cells)
+    p.vol = p.dx * p.dx / 4
+    p.mass = rho * p.vol
+    p.dt = dt
+    p.substeps = substeps
+    p.sort_interval = max(1, substeps // 2)
+    p.mode = mode
+    p.block_x, p.block_y = block
+    if capacity:
+        p.capacity = capacity
+    return p
+
+
+@pytest.mark.parametrize("mode", MODES, ids=lambda m: MODE_NAMES[m])
+def test_free_fall_single_particle(mode):
+    """Isolated fluid particle at rest: v_y = n g dt exactly up to FP32 rounding."""
+    p = small_params(substeps=10, mode=mode, dt=1e-3)
+    out = run_gpu(p, np.array([[0.5, 0.8]], np.float32), material=np.array([abi.MAT_FLUID], np.uint8))
+    assert out["v"][0, 1] == pytest.approx(10 * p.gravity_y * p.dt, rel=1e-5)
+    assert out["v"][0, 0] == 0.0
+    y = 0.8 + p.dt * p.dt * p.gravity_y * 10 * 11 / 2
+    assert out["x"][0, 1] == pytest.approx(y, abs=1e-6)
+
+
+@pytest.mark.parametrize("mode", MODES, ids=lambda m: MODE_NAMES[m])
+@pytest.mark.parametrize("substeps", [1, 10])
+def test_mixed_materials_small(oracle_mod, mode, substeps):
+    """2000 particles of all three materials with random v, C and F near I on a 64-cell grid."""
+    rng = np.random.default_rng(100 + substeps)
+    n = 2000
+    x = rng.uniform(0.3, 0.7, (n, 2)).astype(np.float32)
+    v = rng.uniform(-1, 1, (n, 2)).astype(np.float32)
+    C = rng.uniform(-2, 2, (n, 4)).astype(np.float32)
+    F = (np.tile([1, 0, 0, 1], (n, 1)) + rng.uniform(-0.02, 0.02, (n, 4))).astype(np.float32)
+    J = rng.uniform(0.95, 1.05, n).astype(np.float32)
+    mat = rng.integers(0, 3, n).astype(np.uint8)
+    p = small_params(substeps=substeps, mode=mode, rho=4.0)
+    p.mu, p.lam = 4166.6665, 2777.7778
+    out = run_gpu(p, x, v, mat, C=C, F=F, J=J)
+    s, grid = run_oracle(oracle_mod, p, x, v, mat, substeps, threads=1, C=C, F=F, J=J)
+    assert rel(out["x"], s.x) < 1e-6
+    assert rel(out["v"], s.v) < 1e-4
+    assert rel(out["C"], s.C) < 1e-3
+    assert rel(out["F"], s.F) < 1e-6
+    assert rel(out["J"], s.J) < 1e-6
+    assert rel(out["grid"][0][0], grid[0]) < 1e-5
+    assert np.array_equal(out["material"], mat)
+
+
+@pytest.mark.parametrize("mode", MODES, ids=lambda m: MODE_NAMES[m])
+def test_jelly_block_100_substeps(oracle_mod, mode):
+    """The 128-cell jelly block (32,768 particles) after 100 substeps against the FP64 oracle."""
+    p = abi.preset(abi.PRESET_JELLY128)
+    p.mode = mode
+    sc = scenes.jelly128()
+    out = run_gpu(p, sc.x, sc.v, sc.material, ticks=1)
+    s, grid = run_oracle(oracle_mod, p, sc.x, sc.v, sc.material, p.substeps)
+    assert rel(out["x"], s.x) < 1e-5
+    assert rel(out["v"], s.v) < 1e-3
+    assert rel(out["F"], s.F) < 1e-5
+    assert rel(out["grid"][0][0], grid[0]) < 1e-4
+
+
+def test_jelly_block_1000_substeps_golden():
+    """Fused engine, 32,768-particle jelly block, 1000 substeps, against the committed FP64 oracle fixture
+    (every 32nd particle); bar: 1e-4 relative position error."""
+    import os
+    path = os.path.join(os.path.dirname(__file__), "golden", "jelly128_1000.npz")
+    ref = np.load(path)
+    p = abi.preset(abi.PRESET_JELLY128)
+    sc = scenes.jelly128()
+    out = run_gpu(p, sc.x, sc.v, sc.material, ticks=1000 // p.substeps)
+    idx = ref["index"]
+    assert rel(out["x"][idx], ref["x"]) < 1e-4
+    assert rel(out["v"][idx], ref["v"]) < 1e-2
+    assert rel(out["J"][idx], ref["J"]) < 1e-4
+
+
+def test_mixed1024_scaled_against_oracle(oracle_mod):
+    """The three-box fluid/jelly/snow scene on the 1024-cell grid at 1/32 of the particles, 25 substeps."""
+    p = abi.preset(abi.PRESET_MIXED1024)
+    p.substeps = 25
+    p.sort_interval = 10
+    sc = scenes.mixed(n=131072)
+    out = run_gpu(p, sc.x, sc.v, sc.material)
+    s, grid = run_oracle(oracle_mod, p, sc.x, sc.v, sc.material, p.substeps)
+    assert rel(out["x"], s.x) < 1e-6
+    assert rel(out["v"], s.v) < 1e-3
+    assert rel(out["F"], s.F) < 1e-5
+    assert rel(out["J"], s.J) < 1e-5
+    assert rel(out["grid"][0][0], grid[0]) < 1e-4
+
+
+@pytest.mark.parametrize("shape", [(8, 8), (16, 8), (16, 16), (32, 16)])
+def test_binning_bit_exact(oracle_mod, shape):
+    """The engine's counting sort equals std::stable_sort on identical FP32 keys, bit for bit, including
+    a shuffled input and several scenes."""
+    p = abi.preset(abi.PRESET_BATCH64X256)
+    p.num_scenes = 4
+    p.block_x, p.block_y = shape
+    rng = np.random.default_rng(7)
+    counts = np.array([5000, 1, 20000, 777], np.int32)
+    n = int(counts.sum())
+    p.capacity = n
+    x = rng.uniform(0, 1, (n, 2)).astype(np.float32)
+    x[:50] = 1.0
+    x[50:100] = 0.0
+    scene_of = np.repeat(np.arange(4), counts).astype(np.int32)
+    with Engine(p) as e:
+        e.set_particles(x, material=np.ones(n, np.uint8), scene_counts=counts)
+        keys, ids = e.debug_bin()
+    ref_keys = oracle_mod.bin_keys(p, x, shape[0], shape[1], scene_of)
+    order = oracle_mod.stable_order(ref_keys)
+    assert np.array_equal(ids, order)
+    assert np.array_equal(keys, ref_keys[order])
+
+
+def test_binning_large_random_order(oracle_mod):
+    """4,194,304 particles in random order (the large scene before its first binning)."""
+    p = abi.preset(abi.PRESET_MIXED1024)
+    sc = scenes.mixed()
+    with Engine(p) as e:
+        e.set_scene(sc)
+        keys, ids = e.debug_bin()
+    ref_keys = oracle_mod.bin_keys(p, sc.x, p.block_x, p.block_y)
+    order = oracle_mod.stable_order(ref_keys)
+    assert np.array_equal(ids, order)
+    assert np.array_equal(keys, ref_keys[order])
+
+
+def test_modes_agree_full_size():
+    """4,194,304 particles, 100 substeps: the tiled and fused paths agree with the reference-structured
+    path within the atomic-order noise, conserve mass and keep every particle finite and in the domain."""
+    p = abi.preset(abi.PRESET_MIXED1024)
+    sc = scenes.mixed()
+    outs = {}
+    for mode in MODES:
+        p.mode = mode
+        outs[mode] = run_gpu(p, sc.x, sc.v, sc.material)
+    ref = outs[abi.MODE_REFERENCE]
+    for mode in (abi.MODE_TILED, abi.MODE_FUSED):
+        o = outs[mode]
+        assert rel(o["x"], ref["x"]) < 1e-6, MODE_NAMES[mode]
+        assert rel(o["v"], ref["v"]) < 1e-3, MODE_NAMES[mode]
+        assert rel(o["F"], ref["F"]) < 1e-5, MODE_NAMES[mode]
+    for mode, o in outs.items():
+        assert np.all(np.isfinite(o["x"])) and np.all(np.isfinite(o["v"]))
+        assert o["x"].min() >= 0.0 and o["x"].max() <= 1.0
+        m = o["grid"][0].astype(np.float64).sum()
+        assert m == pytest.approx(sc.n * p.mass, rel=1e-4)
+
+
+def test_stress_kernel_against_oracle(oracle_mod):
+    """Device fixed-corotated / snow / Tait stress against the FP64 oracle, including near-degenerate F
+    (det F < 0.1, the FP64 polar path) and inverted F."""
+    from test_oracle import rand_F
+    p = abi.preset(abi.PRESET_MIXED1024)
+    rng = np.random.default_rng(21)
+    n = 4000
+    F = rand_F(rng, n, lo=0.6, hi=1.6, allow_inverted=True)
+    F[:200] = rand_F(rng, 200, lo=0.05, hi=0.3)
+    mat = rng.integers(0, 3, n).astype(np.uint8)
+    J = rng.uniform(0.8, 1.2, n)
+    with Engine(p) as e:
+        tau = e.compute_stress(mat, F.astype(np.float32), J.astype(np.float32))
+    ref = oracle_mod.tau(p, mat, F.astype(np.float32).astype(np.float64), J.astype(np.float32).astype(np.float64))
+    ref3 = ref[:, [0, 1, 3]]
+    scale = np.abs(ref3).max(axis=1, keepdims=True) + p.mu * 1e-3
+    assert np.max(np.abs(tau - ref3) / scale) < 1e-4
+
+
+def test_output_view_and_emitter():
+    """Hose-pipe emission appends width*rows particles (MPM.cu:450-453) and the output slot written by a
+    tick holds the state in caller order (MPMOutputKernel, MPM.cu:302-322)."""
+    import ctypes
+    p = abi.preset(abi.PRESET_REFERENCE)
+    with Engine(p) as e:
+        e.set_particles(np.zeros((0, 2), np.float32))
+        emitted = e.hose_pipe((30.0, 30.0), (12.0, 0.0), 30.0, 1.26, 0.252, 0)
+        assert emitted == 24 * 10
+        assert e.count() == 240
+        e.step(1)
+        e.sync()
+        out = e.readback()
+        assert np.all(np.isfinite(out["x"]))
+        view = e.read_view()
+        assert view.slot == 0 and view.count == 240
+        hip = ctypes.CDLL("libamdhip64.so")
+        host = np.zeros((240, 2), np.float32)
+        rc = hip.hipMemcpy(host.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(view.pos), ctypes.c_size_t(host.nbytes),
+                           2)
+        assert rc == 0
+        assert np.array_equal(host, out["x"])
+        nums = np.zeros(1, np.int32)
+        hip.hipMemcpy(nums.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(view.num_atoms), ctypes.c_size_t(4), 2)
+        assert nums[0] == 240
+        # second emission uses the committed count as base
+        e.hose_pipe((30.0, 60.0), (12.0, 0.0), 30.0, 1.26, 0.252, 1)
+        assert e.count() == 480
+
+
+def test_capacity_is_respected():
+    p = abi.preset(abi.PRESET_REFERENCE)
+    p.capacity = 300
+    with Engine(p) as e:
+        e.set_particles(np.zeros((0, 2), np.float32))
+        e.hose_pipe((30.0, 30.0), (12.0, 0.0), 30.0, 1.26, 0.0, 0)
+        e.hose_pipe((30.0, 60.0), (12.0, 0.0), 30.0, 1.26, 0.0, 1)
+        assert e.count() == 300
+        e.step(1)
+        e.sync()
+        assert np.all(np.isfinite(e.readback()["x"]))
+
+
+def test_batched_scenes_isolated(oracle_mod):
+    """Scenes of one batched handle evolve independently: each equals the FP64 oracle run alone."""
+    p = abi.preset(abi.PRESET_BATCH64X256)
+    p.num_scenes = 4
+    p.substeps = 20
+    p.sort_interval = 10
+    sc = scenes.batched(num_scenes=4, scale=0.05)
+    p.capacity = sc.n
+    out = run_gpu(p, sc.x, sc.v, sc.material, scene_counts=sc.scene_counts)
+    for k, sl in enumerate(sc.scene_slices()):
+        s, grid = run_oracle(oracle_mod, p, sc.x[sl], sc.v[sl], sc.material[sl], p.substeps, threads=4)
+        assert rel(out["x"][sl], s.x) < 1e-6, k
+        assert rel(out["grid"][0][k], grid[0]) < 1e-4, k
+
+
+def test_time_kernels_advances_like_step(oracle_mod):
+    """The eager, event-timed launch sequence computes the same substeps as the captured tick graph."""
+    p = small_params(substeps=10, mode=abi.MODE_FUSED, rho=4.0)
+    rng = np.random.default_rng(5)
+    x = rng.uniform(0.3, 0.6, (3000, 2)).astype(np.float32)
+    v = rng.uniform(-1, 1, (3000, 2)).astype(np.float32)
+    mat = np.ones(3000, np.uint8)
+    with Engine(p) as e:
+        e.set_particles(x, v, material=mat)
+        t = e.time_kernels(10)
+        a = e.readback()
+    b = run_gpu(p, x, v, mat)
+    assert rel(a["x"], b["x"]) < 1e-6
+    assert t["g2p2g"][1] == 8 and t["binning"][1] == 2 and t["grid_update"][1] == 10
+    assert all(ms >= 0 for ms, _ in t.values())
