@@ -58,10 +58,6 @@ __device__ __forceinline__ int scene_of_key(const DevParams& p, uint32_t key)
 }
 
 __device__ __forceinline__ void atomic_add_global(float* a, float v) { unsafeAtomicAdd(a, v); }
-__device__ __forceinline__ void atomic_add_lds(float* a, float v)
-{
-    __hip_atomic_fetch_add(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
 
 // ===================================================================================== grid update
 
@@ -574,36 +570,41 @@ __device__ __forceinline__ void load_vel_tile(const DevParams& p, const Grid& g,
     }
 }
 
+// LDS accumulators of P2G: one float4 (momentum x, momentum y, mass, unused) per tile node and per wave.
+// Each wave owns its tile, so no two waves ever update the same LDS word; the tiles are summed in a fixed
+// wave order at the flush.
 template <int BX, int BY>
-__device__ __forceinline__ void zero_acc_tile(float* sax, float* say, float* sam)
+__device__ __forceinline__ void zero_acc_tiles(float4* acc)
 {
     using T = TileShape<BX, BY>;
-    for (int t = threadIdx.x; t < T::N; t += kThreads) {
-        sax[t] = 0.f;
-        say[t] = 0.f;
-        sam[t] = 0.f;
-    }
+    for (int t = threadIdx.x; t < T::N * (kThreads / 64); t += kThreads) { acc[t] = make_float4(0.f, 0.f, 0.f, 0.f); }
 }
 
-// One HBM atomic per touched tile node (the LDS tile already reduced every particle of the item).
+// One HBM atomic per touched tile node and component (the LDS tiles already reduced every particle of the
+// work item).
 template <int BX, int BY>
-__device__ __forceinline__ void flush_acc_tile(const DevParams& p, const Grid& g, size_t gb, int ox, int oy,
-                                               const float* sax, const float* say, const float* sam)
+__device__ __forceinline__ void flush_acc_tiles(const DevParams& p, const Grid& g, size_t gb, int ox, int oy,
+                                                const float4* acc)
 {
     using T = TileShape<BX, BY>;
     for (int t = threadIdx.x; t < T::N; t += kThreads) {
-        const float m = sam[t];
-        const float mx = sax[t];
-        const float my = say[t];
-        if (m == 0.f && mx == 0.f && my == 0.f) { continue; }
+        float4 v = acc[t];
+#pragma unroll
+        for (int w = 1; w < kThreads / 64; ++w) {
+            const float4 u = acc[w * T::N + t];
+            v.x += u.x;
+            v.y += u.y;
+            v.z += u.z;
+        }
+        if (v.z == 0.f && v.x == 0.f && v.y == 0.f) { continue; }
         const int ty = t / T::TX;
         const int tx = t - ty * T::TX;
         const int gx = ox + tx, gy = oy + ty;
         if (gx < 0 || gx >= p.W || gy < 0 || gy >= p.H) { continue; }
         const size_t node = gb + (size_t)gy * p.W + gx;
-        atomic_add_global(&g.mom_x[node], mx);
-        atomic_add_global(&g.mom_y[node], my);
-        atomic_add_global(&g.mass[node], m);
+        atomic_add_global(&g.mom_x[node], v.x);
+        atomic_add_global(&g.mom_y[node], v.y);
+        atomic_add_global(&g.mass[node], v.z);
     }
 }
 
@@ -648,12 +649,29 @@ __device__ __forceinline__ void gather(const DevParams& p, const Grid& g, size_t
     }
 }
 
-// Scatter of one particle into the LDS accumulator tile (or HBM when it drifted out of the tile).
+// Bits needed to name a tile node.
+template <int N>
+struct CeilLog2 {
+    static constexpr int value = (N <= 1) ? 0 : 1 + CeilLog2<(N + 1) / 2>::value;
+};
+template <>
+struct CeilLog2<1> {
+    static constexpr int value = 0;
+};
+
+// Scatter of one particle into its wave's LDS accumulator tile, without atomics.  Lanes of the wave are
+// ranked among the lanes whose stencil starts at the same node (an exact multisplit over the node index
+// with one ballot per bit); the lanes of one rank start at distinct nodes, so for each of the 9 stencil
+// offsets their targets are distinct and a plain read-add-write is race free.  Ranks are processed in
+// ascending order and the 9 offsets in a fixed order (the compiler barrier keeps every read after the
+// previous write of the same wave; LDS executes a wave's instructions in order), so the summation order
+// is fixed by the particle order.  A particle that drifted out of the tile scatters to HBM atomically.
 template <int BX, int BY>
-__device__ __forceinline__ void scatter(const DevParams& p, const Grid& g, size_t gb, int ox, int oy, float* sax,
-                                        float* say, float* sam, float px, float py, float vx, float vy, const Mat2& Q)
+__device__ __forceinline__ void scatter(const DevParams& p, const Grid& g, size_t gb, int ox, int oy, float4* wtile,
+                                        float px, float py, float vx, float vy, const Mat2& Q)
 {
     using T = TileShape<BX, BY>;
+    constexpr int kBits = CeilLog2<T::N>::value;
     int bx, by;
     float fx, fy, wx[3], wy[3];
     bspline(px, p.inv_dx, (float)p.halo, bx, fx, wx);
@@ -663,6 +681,8 @@ __device__ __forceinline__ void scatter(const DevParams& p, const Grid& g, size_
     const int lx = bx - ox, ly = by - oy;
     const bool inside = (unsigned)lx <= (unsigned)(T::TX - 3) && (unsigned)ly <= (unsigned)(T::TY - 3);
     const float mvx = vx * p.mass, mvy = vy * p.mass;
+    // contribution to stencil node (a, b): w_ab * (m v + Q d_ab, m), d_ab = ((a - fx) dx, (b - fy) dx)
+    float cx[9], cy[9], cm[9];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
 #pragma unroll
@@ -672,19 +692,51 @@ __device__ __forceinline__ void scatter(const DevParams& p, const Grid& g, size_
             const float dpy = ((float)b - fy) * p.dx;
             const float afx = dpx * Q.a00 + dpy * Q.a01;
             const float afy = dpx * Q.a10 + dpy * Q.a11;
-            const float ax = w * (mvx + afx), ay = w * (mvy + afy), am = w * p.mass;
-            if (inside) {
-                const int t = (ly + b) * T::TX + lx + a;
-                atomic_add_lds(&sax[t], ax);
-                atomic_add_lds(&say[t], ay);
-                atomic_add_lds(&sam[t], am);
-            } else {
+            cx[a * 3 + b] = w * (mvx + afx);
+            cy[a * 3 + b] = w * (mvy + afy);
+            cm[a * 3 + b] = w * p.mass;
+        }
+    }
+    if (!inside) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+#pragma unroll
+            for (int b = 0; b < 3; ++b) {
                 const size_t node = gb + (size_t)(by + b) * p.W + (bx + a);
-                atomic_add_global(&g.mom_x[node], ax);
-                atomic_add_global(&g.mom_y[node], ay);
-                atomic_add_global(&g.mass[node], am);
+                atomic_add_global(&g.mom_x[node], cx[a * 3 + b]);
+                atomic_add_global(&g.mom_y[node], cy[a * 3 + b]);
+                atomic_add_global(&g.mass[node], cm[a * 3 + b]);
             }
         }
+    }
+    const int base = ly * T::TX + lx;
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    uint64_t peers = __ballot(inside);
+#pragma unroll
+    for (int k = 0; k < kBits; ++k) {
+        const bool bit = (base >> k) & 1;
+        const uint64_t m = __ballot(inside && bit);
+        peers &= bit ? m : ~m;
+    }
+    const int rank = __popcll(peers & lt_mask);
+    for (int r = 0;; ++r) {
+        if (inside && rank == r) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+#pragma unroll
+                for (int b = 0; b < 3; ++b) {
+                    float4* cell = &wtile[base + b * T::TX + a];
+                    float4 v = *cell;
+                    v.x += cx[a * 3 + b];
+                    v.y += cy[a * 3 + b];
+                    v.z += cm[a * 3 + b];
+                    *cell = v;
+                    __asm__ __volatile__("" ::: "memory");
+                }
+            }
+        }
+        if (__ballot(inside && rank > r) == 0ull) { break; }
     }
 }
 
@@ -694,7 +746,7 @@ __global__ __launch_bounds__(kThreads) void k_p2g_tiled(DevParams p, Store s, Gr
                                                         const int32_t* num_items, const int32_t* count)
 {
     using T = TileShape<BX, BY>;
-    __shared__ float sax[T::N], say[T::N], sam[T::N];
+    __shared__ float4 acc[T::N * (kThreads / 64)];
     bool ok;
     const int it = xcd_item(num_items[0], ok);
     if (!ok) { return; }
@@ -703,14 +755,15 @@ __global__ __launch_bounds__(kThreads) void k_p2g_tiled(DevParams p, Store s, Gr
     int scene, ox, oy;
     item_origin(p, item.key, BX, BY, scene, ox, oy);
     const size_t gb = (size_t)scene * (size_t)p.G;
-    zero_acc_tile<BX, BY>(sax, say, sam);
+    zero_acc_tiles<BX, BY>(acc);
     __syncthreads();
+    float4* wtile = acc + (threadIdx.x >> 6) * T::N;
     for (int j = item.begin + threadIdx.x; j < item.end; j += kThreads) {
         const Mat2 Q = affine_q(p, s.mat[j], load_C(s, j), load_F(s, j), s.J[j]);
-        scatter<BX, BY>(p, g, gb, ox, oy, sax, say, sam, s.x[j], s.y[j], s.vx[j], s.vy[j], Q);
+        scatter<BX, BY>(p, g, gb, ox, oy, wtile, s.x[j], s.y[j], s.vx[j], s.vy[j], Q);
     }
     __syncthreads();
-    flush_acc_tile<BX, BY>(p, g, gb, ox, oy, sax, say, sam);
+    flush_acc_tiles<BX, BY>(p, g, gb, ox, oy, acc);
 }
 
 // G2P over binned particles, velocities staged in an LDS tile.
@@ -748,7 +801,7 @@ __global__ __launch_bounds__(kThreads) void k_g2p_tiled(DevParams p, Store s, Gr
 }
 
 // G2P of substep s fused with P2G of substep s+1: the particle is read once (x, F, J, material), gathers
-// from the LDS velocity tile of grid s, is updated, and scatters into the LDS accumulator tile of grid
+// from the LDS velocity tile of grid s, is updated, and scatters into the LDS accumulator tiles of grid
 // s+1, flushed with one HBM atomic per touched node.  v and C stay in registers (they are recomputed
 // by every G2P); they are written only by the last G2P of a segment (k_g2p_tiled).
 template <int BX, int BY>
@@ -757,7 +810,7 @@ __global__ __launch_bounds__(kThreads) void k_g2p2g(DevParams p, Store s, Grid g
 {
     using T = TileShape<BX, BY>;
     __shared__ float2 sv[T::N];
-    __shared__ float sax[T::N], say[T::N], sam[T::N];
+    __shared__ float4 acc[T::N * (kThreads / 64)];
     bool ok;
     const int it = xcd_item(num_items[0], ok);
     if (!ok) { return; }
@@ -767,8 +820,9 @@ __global__ __launch_bounds__(kThreads) void k_g2p2g(DevParams p, Store s, Grid g
     item_origin(p, item.key, BX, BY, scene, ox, oy);
     const size_t gb = (size_t)scene * (size_t)p.G;
     load_vel_tile<BX, BY>(p, g, gb, ox, oy, sv);
-    zero_acc_tile<BX, BY>(sax, say, sam);
+    zero_acc_tiles<BX, BY>(acc);
     __syncthreads();
+    float4* wtile = acc + (threadIdx.x >> 6) * T::N;
     for (int j = item.begin + threadIdx.x; j < item.end; j += kThreads) {
         float px = s.x[j], py = s.y[j], vx, vy;
         const int mat = s.mat[j];
@@ -782,10 +836,10 @@ __global__ __launch_bounds__(kThreads) void k_g2p2g(DevParams p, Store s, Grid g
         store_F(s, j, F);
         s.J[j] = J;
         const Mat2 Q = affine_q(p, mat, C, F, J);
-        scatter<BX, BY>(p, g, gb, ox, oy, sax, say, sam, px, py, vx, vy, Q);
+        scatter<BX, BY>(p, g, gb, ox, oy, wtile, px, py, vx, vy, Q);
     }
     __syncthreads();
-    flush_acc_tile<BX, BY>(p, g, gb, ox, oy, sax, say, sam);
+    flush_acc_tiles<BX, BY>(p, g, gb, ox, oy, acc);
 }
 
 int shape_index(int bx, int by)

@@ -81,7 +81,13 @@ MIXED_BOXES = (
 
 def mixed(n=4194304, materials=(abi.MAT_FLUID, abi.MAT_JELLY, abi.MAT_SNOW), seed=SEED_BASE + abi.PRESET_MIXED1024,
           boxes=MIXED_BOXES):
-    """n particles split evenly over three boxes, initial velocity uniform in [-1,1]^2."""
+    """n particles split evenly over three boxes, initial velocity uniform in [-1,1]^2.  For n below
+    4,194,304 every box shrinks about its centre by sqrt(n / 4194304), keeping 8 particles per cell."""
+    if n < 4194304:
+        f = np.sqrt(n / 4194304.0)
+        boxes = tuple((((lo[0] + hi[0]) / 2 - f * (hi[0] - lo[0]) / 2, (lo[1] + hi[1]) / 2 - f * (hi[1] - lo[1]) / 2),
+                       ((lo[0] + hi[0]) / 2 + f * (hi[0] - lo[0]) / 2, (lo[1] + hi[1]) / 2 + f * (hi[1] - lo[1]) / 2))
+                      for lo, hi in boxes)
     parts_x, parts_m = [], []
     counts = [n // 3, n // 3, n - 2 * (n // 3)]
     off = 0

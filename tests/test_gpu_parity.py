@@ -128,7 +128,7 @@ def test_mixed1024_scaled_against_oracle(oracle_mod):
     p = abi.preset(abi.PRESET_MIXED1024)
     p.substeps = 25
     p.sort_interval = 10
-    sc = scenes.mixed(n=131072)
+    sc = scenes.mixed(n=131072)  # 8 particles per cell in shrunken boxes
     out = run_gpu(p, sc.x, sc.v, sc.material)
     s, grid = run_oracle(oracle_mod, p, sc.x, sc.v, sc.material, p.substeps)
     assert rel(out["x"], s.x) < 1e-6

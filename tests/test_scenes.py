@@ -28,6 +28,10 @@ def test_mixed_scene_layout():
         sel = s.x[s.material == m]
         assert sel[:, 0].min() >= lo[0] and sel[:, 0].max() <= hi[0]
         assert sel[:, 1].min() >= lo[1] and sel[:, 1].max() <= hi[1]
+    # density kept at 8 particles per cell of the 1024-cell grid
+    sel = s.x[s.material == 1]
+    area = np.ptp(sel[:, 0]) * np.ptp(sel[:, 1]) * 1024 ** 2
+    assert abs(len(sel) / area - 8.0) < 0.5
 
 
 def test_batched_counts_log_uniform():
