@@ -58,6 +58,23 @@ def cpu_baseline(params, scene, budget_s=12.0):
                       f"({dt:.1f} s) after 1 untimed substep"}
 
 
+def copy_calibration(mib, reps=5):
+    """Device-to-device copies of a known size (hipMemcpy), so profiler byte counters can be scaled to HBM
+    bytes: each copy reads and writes exactly mib MiB."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = ctypes.c_size_t(mib << 20)
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), n) == 0 and hip.hipMalloc(ctypes.byref(b), n) == 0
+    hip.hipMemset(a, 1, n)
+    for _ in range(reps):
+        assert hip.hipMemcpy(b, a, n, 3) == 0  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"copy calibration: {reps} device copies of {mib} MiB")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -66,7 +83,11 @@ def main():
     ap.add_argument("--mode", choices=["reference", "tiled", "fused"], default="fused")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--scale", type=float, default=1.0, help="particle count scale (1.0 = 4,194,304)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="also run device copies of MIB MiB (profiler counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported (multi-GPU sharding "
               f"is a design note, not built)", file=sys.stderr)

@@ -135,6 +135,41 @@ __device__ __forceinline__ void store_C(const Store& s, int i, const Mat2& C)
     s.c11[i] = C.a11;
 }
 
+// SoA element access through the store's base pointer and a 32-bit byte offset: the loads and stores
+// use the SGPR-base + VGPR-offset form, with no 64-bit address arithmetic per component.
+template <class T>
+__device__ __forceinline__ T ldo(const T* base, uint32_t off)
+{
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off);
+}
+template <class T>
+__device__ __forceinline__ void sto(T* base, uint32_t off, T v)
+{
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + off) = v;
+}
+__device__ __forceinline__ Mat2 ld_F(const Store& s, uint32_t o)
+{
+    return {ldo(s.f00, o), ldo(s.f01, o), ldo(s.f10, o), ldo(s.f11, o)};
+}
+__device__ __forceinline__ Mat2 ld_C(const Store& s, uint32_t o)
+{
+    return {ldo(s.c00, o), ldo(s.c01, o), ldo(s.c10, o), ldo(s.c11, o)};
+}
+__device__ __forceinline__ void st_F(const Store& s, uint32_t o, const Mat2& F)
+{
+    sto(s.f00, o, F.a00);
+    sto(s.f01, o, F.a01);
+    sto(s.f10, o, F.a10);
+    sto(s.f11, o, F.a11);
+}
+__device__ __forceinline__ void st_C(const Store& s, uint32_t o, const Mat2& C)
+{
+    sto(s.c00, o, C.a00);
+    sto(s.c01, o, C.a01);
+    sto(s.c10, o, C.a10);
+    sto(s.c11, o, C.a11);
+}
+
 // ===================================================================================== reference structure
 
 // P2GKernel + particleToGrid (MPM.cu:53-129): one particle per lane, 27 float atomics into HBM.
@@ -608,7 +643,41 @@ __device__ __forceinline__ void flush_acc_tiles(const DevParams& p, const Grid& 
     }
 }
 
-// Gather of one particle from the LDS velocity tile (or HBM when it drifted out of the tile).
+// Gather of one particle (MPM.cu:227-251) from the LDS velocity tile, or from HBM when it drifted out of
+// the tile.  v = sum w_ab v_ab, and with d_ab = ((a - fx) dx, (b - fy) dx) the affine moment
+// B = sum w_ab v_ab (x) d_ab = dx (sum w_ab v_ab (a, b) - v (x) f): two first moments replace the
+// per-node outer products (each column of three nodes is reduced with wy first).
+template <class Load>
+__device__ __forceinline__ void gather_moments(const float wx[3], const float wy[3], Load load, float& sx, float& sy,
+                                               float& ax, float& ay, float& bx, float& by)
+{
+    sx = sy = ax = ay = bx = by = 0.f;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        float cx = 0.f, cy = 0.f, dx = 0.f, dy = 0.f;
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            const float2 gv = load(a, b);
+            cx = fmaf(wy[b], gv.x, cx);
+            cy = fmaf(wy[b], gv.y, cy);
+            if (b > 0) {
+                const float wb = wy[b] * (float)b;
+                dx = fmaf(wb, gv.x, dx);
+                dy = fmaf(wb, gv.y, dy);
+            }
+        }
+        sx = fmaf(wx[a], cx, sx);
+        sy = fmaf(wx[a], cy, sy);
+        bx = fmaf(wx[a], dx, bx);
+        by = fmaf(wx[a], dy, by);
+        if (a > 0) {
+            const float wa = wx[a] * (float)a;
+            ax = fmaf(wa, cx, ax);
+            ay = fmaf(wa, cy, ay);
+        }
+    }
+}
+
 template <int BX, int BY>
 __device__ __forceinline__ void gather(const DevParams& p, const Grid& g, size_t gb, int ox, int oy, const float2* sv,
                                        float px, float py, float& vx, float& vy, Mat2& B)
@@ -622,31 +691,17 @@ __device__ __forceinline__ void gather(const DevParams& p, const Grid& g, size_t
     by = clampi(by, 0, p.H - 3);
     const int lx = bx - ox, ly = by - oy;
     const bool inside = (unsigned)lx <= (unsigned)(T::TX - 3) && (unsigned)ly <= (unsigned)(T::TY - 3);
-    vx = 0.f;
-    vy = 0.f;
-    B = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-#pragma unroll
-        for (int b = 0; b < 3; ++b) {
-            const float w = wx[a] * wy[b];
-            const float dpx = ((float)a - fx) * p.dx;
-            const float dpy = ((float)b - fy) * p.dx;
-            float2 gv;
-            if (inside) {
-                gv = sv[(ly + b) * T::TX + lx + a];
-            } else {
-                gv = g.vel[gb + (size_t)(by + b) * p.W + (bx + a)];
-            }
-            vx += w * gv.x;
-            vy += w * gv.y;
-            const float wvx = w * gv.x, wvy = w * gv.y;
-            B.a00 += wvx * dpx;
-            B.a01 += wvx * dpy;
-            B.a10 += wvy * dpx;
-            B.a11 += wvy * dpy;
-        }
+    float sx, sy, mx, my, nx, ny;
+    if (inside) {
+        const float2* t0 = sv + ly * T::TX + lx;
+        gather_moments(wx, wy, [&](int a, int b) { return t0[b * T::TX + a]; }, sx, sy, mx, my, nx, ny);
+    } else {
+        const float2* g0 = g.vel + gb + (size_t)by * p.W + bx;
+        gather_moments(wx, wy, [&](int a, int b) { return g0[(size_t)b * p.W + a]; }, sx, sy, mx, my, nx, ny);
     }
+    vx = sx;
+    vy = sy;
+    B = {p.dx * (mx - fx * sx), p.dx * (nx - fy * sx), p.dx * (my - fx * sy), p.dx * (ny - fy * sy)};
 }
 
 // Bits needed to name a tile node.
@@ -680,22 +735,28 @@ __device__ __forceinline__ void scatter(const DevParams& p, const Grid& g, size_
     by = clampi(by, 0, p.H - 3);
     const int lx = bx - ox, ly = by - oy;
     const bool inside = (unsigned)lx <= (unsigned)(T::TX - 3) && (unsigned)ly <= (unsigned)(T::TY - 3);
-    const float mvx = vx * p.mass, mvy = vy * p.mass;
-    // contribution to stencil node (a, b): w_ab * (m v + Q d_ab, m), d_ab = ((a - fx) dx, (b - fy) dx)
+    // contribution to stencil node (a, b): w_ab (m v + Q d_ab, m) with d_ab = ((a - fx) dx, (b - fy) dx),
+    // i.e. w_ab (A0 + a dx Q.col0 + b dx Q.col1, m), A0 = m v - dx Q f, built incrementally over a and b.
+    const float qax = p.dx * Q.a00, qay = p.dx * Q.a10, qbx = p.dx * Q.a01, qby = p.dx * Q.a11;
+    const float a0x = vx * p.mass - (qax * fx + qbx * fy);
+    const float a0y = vy * p.mass - (qay * fx + qby * fy);
+    const float wym[3] = {wy[0] * p.mass, wy[1] * p.mass, wy[2] * p.mass};
     float cx[9], cy[9], cm[9];
+    float tax = a0x, tay = a0y;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
+        float tx = tax, ty = tay;
 #pragma unroll
         for (int b = 0; b < 3; ++b) {
             const float w = wx[a] * wy[b];
-            const float dpx = ((float)a - fx) * p.dx;
-            const float dpy = ((float)b - fy) * p.dx;
-            const float afx = dpx * Q.a00 + dpy * Q.a01;
-            const float afy = dpx * Q.a10 + dpy * Q.a11;
-            cx[a * 3 + b] = w * (mvx + afx);
-            cy[a * 3 + b] = w * (mvy + afy);
-            cm[a * 3 + b] = w * p.mass;
+            cx[a * 3 + b] = w * tx;
+            cy[a * 3 + b] = w * ty;
+            cm[a * 3 + b] = wx[a] * wym[b];
+            tx += qbx;
+            ty += qby;
         }
+        tax += qax;
+        tay += qay;
     }
     if (!inside) {
 #pragma unroll
@@ -759,8 +820,9 @@ __global__ __launch_bounds__(kThreads) void k_p2g_tiled(DevParams p, Store s, Gr
     __syncthreads();
     float4* wtile = acc + (threadIdx.x >> 6) * T::N;
     for (int j = item.begin + threadIdx.x; j < item.end; j += kThreads) {
-        const Mat2 Q = affine_q(p, s.mat[j], load_C(s, j), load_F(s, j), s.J[j]);
-        scatter<BX, BY>(p, g, gb, ox, oy, wtile, s.x[j], s.y[j], s.vx[j], s.vy[j], Q);
+        const uint32_t o = (uint32_t)j * 4u;
+        const Mat2 Q = affine_q(p, ldo(s.mat, (uint32_t)j), ld_C(s, o), ld_F(s, o), ldo(s.J, o));
+        scatter<BX, BY>(p, g, gb, ox, oy, wtile, ldo(s.x, o), ldo(s.y, o), ldo(s.vx, o), ldo(s.vy, o), Q);
     }
     __syncthreads();
     flush_acc_tiles<BX, BY>(p, g, gb, ox, oy, acc);
@@ -784,19 +846,20 @@ __global__ __launch_bounds__(kThreads) void k_g2p_tiled(DevParams p, Store s, Gr
     load_vel_tile<BX, BY>(p, g, gb, ox, oy, sv);
     __syncthreads();
     for (int j = item.begin + threadIdx.x; j < item.end; j += kThreads) {
-        float px = s.x[j], py = s.y[j], vx, vy;
-        Mat2 B;
+        const uint32_t o = (uint32_t)j * 4u;
+        float px = ldo(s.x, o), py = ldo(s.y, o), vx, vy;
+        const int mat = ldo(s.mat, (uint32_t)j);
+        Mat2 C, B, F = ld_F(s, o);
+        float J = ldo(s.J, o);
         gather<BX, BY>(p, g, gb, ox, oy, sv, px, py, vx, vy, B);
-        Mat2 C, F = load_F(s, j);
-        float J = s.J[j];
-        g2p_update(p, s.mat[j], px, py, vx, vy, B, C, F, J);
-        s.x[j] = px;
-        s.y[j] = py;
-        s.vx[j] = vx;
-        s.vy[j] = vy;
-        store_C(s, j, C);
-        store_F(s, j, F);
-        s.J[j] = J;
+        g2p_update(p, mat, px, py, vx, vy, B, C, F, J);
+        sto(s.x, o, px);
+        sto(s.y, o, py);
+        sto(s.vx, o, vx);
+        sto(s.vy, o, vy);
+        st_C(s, o, C);
+        st_F(s, o, F);
+        sto(s.J, o, J);
     }
 }
 
@@ -824,17 +887,17 @@ __global__ __launch_bounds__(kThreads) void k_g2p2g(DevParams p, Store s, Grid g
     __syncthreads();
     float4* wtile = acc + (threadIdx.x >> 6) * T::N;
     for (int j = item.begin + threadIdx.x; j < item.end; j += kThreads) {
-        float px = s.x[j], py = s.y[j], vx, vy;
-        const int mat = s.mat[j];
-        Mat2 B;
+        const uint32_t o = (uint32_t)j * 4u;
+        float px = ldo(s.x, o), py = ldo(s.y, o), vx, vy;
+        const int mat = ldo(s.mat, (uint32_t)j);
+        Mat2 C, B, F = ld_F(s, o);
+        float J = ldo(s.J, o);
         gather<BX, BY>(p, g, gb, ox, oy, sv, px, py, vx, vy, B);
-        Mat2 C, F = load_F(s, j);
-        float J = s.J[j];
         g2p_update(p, mat, px, py, vx, vy, B, C, F, J);
-        s.x[j] = px;
-        s.y[j] = py;
-        store_F(s, j, F);
-        s.J[j] = J;
+        sto(s.x, o, px);
+        sto(s.y, o, py);
+        st_F(s, o, F);
+        sto(s.J, o, J);
         const Mat2 Q = affine_q(p, mat, C, F, J);
         scatter<BX, BY>(p, g, gb, ox, oy, wtile, px, py, vx, vy, Q);
     }

@@ -121,7 +121,7 @@ __device__ __forceinline__ void polar_cs_f32(const Mat2& F, float& c, float& s)
     }
 }
 // Same in FP64, used only for near-degenerate F (det F < 0.1 or det F > 10).
-__device__ __noinline__ void polar_cs_f64(const Mat2& F, float& c, float& s)
+__device__ __forceinline__ void polar_cs_f64(const Mat2& F, float& c, float& s)
 {
     const double x = (double)F.a00 + (double)F.a11;
     const double y = (double)F.a10 - (double)F.a01;
@@ -180,8 +180,8 @@ __device__ __forceinline__ void svd2(const Mat2& F, float& uc, float& us, float&
     const float off = 0.5f * (S01 + S10);
     float jc = 1.f, js = 0.f;
     if (off != 0.f) {
-        const float tau = 0.5f * (S11 - S00) / off;
-        const float t = copysignf(1.f, tau) / (fabsf(tau) + sqrtf(1.f + tau * tau));
+        const float tau = 0.5f * (S11 - S00) * __builtin_amdgcn_rcpf(off);
+        const float t = copysignf(__builtin_amdgcn_rcpf(fabsf(tau) + sqrtf(1.f + tau * tau)), tau);
         jc = rsqrtf(1.f + t * t);
         js = t * jc;
     }
@@ -220,7 +220,9 @@ __device__ __forceinline__ void snow_project(const DevParams& p, Mat2& F, float&
 __device__ __forceinline__ Mat2 affine_q(const DevParams& p, int mat, const Mat2& C, const Mat2& F, float J)
 {
     if (mat == MPM_MAT_FLUID) {
-        const float p_hat = p.tait_k * (powf(1.f / J, p.tait_gamma) - 1.f);
+        // (1/J)^gamma; gamma = 1 (linear bulk modulus) skips powf, whose result would be 1/J exactly.
+        const float inv_j = 1.f / J;
+        const float p_hat = p.tait_k * ((p.tait_gamma == 1.f ? inv_j : powf(inv_j, p.tait_gamma)) - 1.f);
         const float pressure = fminf(fmaxf(p_hat, p.p_min), p.p_max);
         const float V = p.vol * J;
         const float coeff = -p.dt * V * 4.f * p.inv_dx * p.inv_dx;
