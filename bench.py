@@ -36,6 +36,31 @@ SUBSTEP_BYTES_PER_PARTICLE = 132
 SUBSTEP_BYTES_PER_NODE = 40
 
 
+WORKLOADS = {
+    # name: (preset, scene factory, description)
+    "mixed1024": (abi.PRESET_MIXED1024, lambda scale: scenes.mixed(n=int(4194304 * scale)),
+                  "mixed1024: 4,194,304 particles (fluid/jelly/snow boxes) on a 1024x1024-cell grid, dt=2e-5, "
+                  "100 substeps per step"),
+    "jelly128": (abi.PRESET_JELLY128, lambda scale: scenes.jelly128(n=int(32768 * scale)),
+                 "jelly128: 32,768-particle jelly block on a 128x128-cell grid, dt=1e-4, 100 substeps per step"),
+    "batch64x256": (abi.PRESET_BATCH64X256, lambda scale: scenes.batched(scale=scale),
+                    "batch64x256: 64 scenes of 256x256 cells, particle counts log-uniform in [4096, 524288], "
+                    "mixed materials, dt=2e-5, 200 substeps per step"),
+}
+
+
+def traffic_from_profile(p, workload):
+    """HBM bytes per launch of k_g2p2g measured by rocprofv3 PMC for this workload and launch shape
+    (profiles/g2p2g_<workload>_traffic.json), or None."""
+    path = os.path.join(ROOT, "profiles", f"g2p2g_{workload}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    d = json.load(open(path))
+    if list(d.get("tile", [])) != [p.block_x, p.block_y] or d.get("chunk") != p.chunk:
+        return None
+    return d["traffic_bytes"]
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -45,6 +70,20 @@ def cpu_baseline(params, scene, budget_s=12.0):
     particle set for as many substeps as fit the budget (at least 1, at most 20)."""
     import oracle
     threads = min(16, len(os.sched_getaffinity(0)))
+    if params.num_scenes > 1:
+        # one scene at a time (each scene has its own grid), as many scenes as fit the budget
+        done, t_all, k = 0, 0.0, 0
+        for k, sl in enumerate(scene.scene_slices()):
+            s = oracle.State(scene.x[sl], scene.v[sl], material=scene.material[sl])
+            t0 = time.perf_counter()
+            oracle.run(params, s, 1, threads=threads)
+            t_all += time.perf_counter() - t0
+            done += sl.stop - sl.start
+            if t_all > budget_s:
+                break
+        return {"value": done / t_all, "unit": "particle-substeps/s", "cores": threads, "kind": "port",
+                "sample": f"FP64 oracle, OpenMP {threads} threads, 1 substep of each of the first {k + 1} scenes "
+                          f"({done} particles, {t_all:.1f} s)"}
     s = oracle.State(scene.x, scene.v, material=scene.material)
     t0 = time.perf_counter()
     oracle.run(params, s, 1, threads=threads)
@@ -83,6 +122,9 @@ def main():
     ap.add_argument("--mode", choices=["reference", "tiled", "fused"], default="fused")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--scale", type=float, default=1.0, help="particle count scale (1.0 = 4,194,304)")
+    ap.add_argument("--autotune", action="store_true", help="pick tile shape, chunk and sort interval by mpm_autotune")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="mixed1024",
+                    help="mixed1024 is the headline; the others are the remaining configurations")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run device copies of MIB MiB (profiler counter calibration)")
     args = ap.parse_args()
@@ -93,11 +135,19 @@ def main():
               f"is a design note, not built)", file=sys.stderr)
         sys.exit(2)
 
-    p = abi.preset(abi.PRESET_MIXED1024)
+    preset, make_scene, description = WORKLOADS[args.workload]
+    p = abi.preset(preset)
     p.mode = {"reference": abi.MODE_REFERENCE, "tiled": abi.MODE_TILED, "fused": abi.MODE_FUSED}[args.mode]
-    scene = scenes.mixed(n=int(4194304 * args.scale))
+    scene = make_scene(args.scale)
     p.capacity = scene.n
     log(f"scene: {scene.description}")
+    if args.autotune and p.mode != abi.MODE_REFERENCE:
+        from mpm_amd.engine import autotune
+        t0 = time.perf_counter()
+        p, res = autotune(p, scene)
+        log(f"autotune: {res.candidates} candidates in {time.perf_counter() - t0:.1f} s -> tile {res.block_x}x"
+            f"{res.block_y}, chunk {res.chunk}, sort interval {res.sort_interval}, "
+            f"{res.ms_per_substep * 1e3:.1f} us/substep")
     eng = Engine(p)
     eng.set_scene(scene)
     log("engine created, state resident in HBM")
@@ -149,14 +199,15 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "mixed1024: 4,194,304 particles (fluid/jelly/snow boxes) on a 1024x1024-cell grid, "
-                               "dt=2e-5, 100 substeps per step" if args.scale == 1.0 else
-                               f"mixed1024 scaled x{args.scale}: {scene.n} particles",
-                   "particles": scene.n, "grid_cells": 1024, "substeps_per_step": p.substeps,
+        "config": {"workload": description if args.scale == 1.0 else
+                               f"{args.workload} scaled x{args.scale}: {scene.n} particles",
+                   "particles": scene.n, "scenes": p.num_scenes, "grid_cells": p.grid_w - 4,
+                   "substeps_per_step": p.substeps,
                    "mode": args.mode, "tile": [p.block_x, p.block_y], "chunk": p.chunk,
                    "sort_interval": p.sort_interval},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic_from_profile(p, args.workload) if dominant == "g2p2g" else None,
                      "bytes_per_launch": bytes_per_launch,
                      "us_per_launch": 1e3 * ms_dom / n_dom if n_dom else None},
         "substep": {"us": substep_s * 1e6, "algorithmic_bytes_unfused": substep_bytes,

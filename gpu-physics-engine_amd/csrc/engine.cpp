@@ -246,12 +246,14 @@ struct LaunchTimer {
     hipStream_t st = nullptr;
     std::vector<hipEvent_t> ev;
     std::vector<int> cls;
+    // Timing events without the system-scope release a default event performs (an L2 write-back that
+    // would otherwise be counted inside the bracketed kernel).
     hipError_t begin(int k)
     {
         hipEvent_t a, b;
-        hipError_t e = hipEventCreate(&a);
+        hipError_t e = hipEventCreateWithFlags(&a, hipEventDisableSystemFence);
         if (e != hipSuccess) { return e; }
-        e = hipEventCreate(&b);
+        e = hipEventCreateWithFlags(&b, hipEventDisableSystemFence);
         if (e != hipSuccess) { return e; }
         ev.push_back(a);
         ev.push_back(b);
@@ -370,6 +372,10 @@ int32_t mpm_params_preset(int32_t preset, MpmParams* out)
     MpmParams p;
     std::memset(&p, 0, sizeof(p));
     // Fields common to the unit-square presets.
+    // rho: particle density.  With fixed-corotated jelly E = 1e4, nu = 0.2 the elastic wave speed is
+    // sqrt((lambda + 2 mu) / rho); rho = 16 keeps c dt / dx at about 0.54 on the 1024-cell grid at
+    // dt = 2e-5 (rho = 1 diverges within 10 substeps; rho = 4, CFL 1.08, amplifies rounding noise over
+    // several hundred substeps).
     auto unit_square = [&](int cells, float dt, float rho) {
         p.grid_w = p.grid_h = cells + 4;
         p.halo = 2;
@@ -395,9 +401,10 @@ int32_t mpm_params_preset(int32_t preset, MpmParams* out)
         p.num_scenes = 1;
         p.mode = MPM_MODE_FUSED;
         p.closure = MPM_CLOSURE_ANALYTIC;
+        // launch shape chosen by mpm_autotune on MI355X for the 4,194,304-particle mixed scene
         p.block_x = 16;
-        p.block_y = 16;
-        p.chunk = 512;
+        p.block_y = 8;
+        p.chunk = 2048;
         p.out_chunk = 1536;
     };
     auto lame = [&](float E, float nu) {
@@ -445,14 +452,14 @@ int32_t mpm_params_preset(int32_t preset, MpmParams* out)
         p.sort_interval = 50;
         break;
     case MPM_PRESET_MIXED1024:
-        unit_square(1024, 2e-5f, 4.f);
+        unit_square(1024, 2e-5f, 16.f);
         lame(1e4f, 0.2f);
         p.substeps = 100;
         p.capacity = 4194304;
-        p.sort_interval = 25;
+        p.sort_interval = 100;
         break;
     case MPM_PRESET_BATCH64X256:
-        unit_square(256, 2e-5f, 4.f);
+        unit_square(256, 2e-5f, 16.f);
         lame(1e4f, 0.2f);
         p.substeps = 200;
         p.num_scenes = 64;
@@ -460,7 +467,7 @@ int32_t mpm_params_preset(int32_t preset, MpmParams* out)
         p.sort_interval = 25;
         break;
     case MPM_PRESET_LEARNED1024:
-        unit_square(1024, 2e-5f, 4.f);
+        unit_square(1024, 2e-5f, 16.f);
         lame(1e4f, 0.2f);
         p.substeps = 100;
         p.capacity = 4194304;
@@ -468,7 +475,7 @@ int32_t mpm_params_preset(int32_t preset, MpmParams* out)
         p.closure = MPM_CLOSURE_LEARNED;
         break;
     case MPM_PRESET_LEARNED16X2048:
-        unit_square(2048, 1e-5f, 4.f);
+        unit_square(2048, 1e-5f, 16.f);
         lame(1e4f, 0.2f);
         p.substeps = 100;
         p.num_scenes = 16;
@@ -747,6 +754,76 @@ int32_t mpm_hose_pipe(MpmEngine* h, float origin_x, float origin_y, float veloci
                                         1.f / tick_len);
     if (e != hipSuccess) { return fail(h, MPM_ERR_DEVICE, (int32_t)e, "emitter launch"); }
     if (emitted) { *emitted = count; }
+    return MPM_OK;
+}
+
+int32_t mpm_autotune(const MpmParams* base, int32_t n, const float* x, const float* v, const uint8_t* material,
+                     const int32_t* scene_counts, int32_t substeps, MpmTuneResult* out)
+{
+    if (!base || !out || substeps < 1 || n < 0 || (n > 0 && !x)) {
+        return fail(nullptr, MPM_ERR_INVALID_ARG, 0, "mpm_autotune: bad arguments");
+    }
+    if (base->mode == MPM_MODE_REFERENCE) { return fail(nullptr, MPM_ERR_STATE, 0, "reference mode has no tiles"); }
+    // Time one configuration: warm-up tick, then one tick measured with the handle's t0/t1 events.
+    auto time_one = [&](const MpmParams& p, float* ms) -> int32_t {
+        MpmEngine* h = nullptr;
+        int32_t rc = mpm_create(&p, &h);
+        if (rc != MPM_OK) { return rc; }
+        rc = mpm_set_particles(h, n, x, v, nullptr, nullptr, nullptr, material, scene_counts);
+        if (rc == MPM_OK) { rc = mpm_step(h, 1); }
+        if (rc == MPM_OK) { rc = mpm_step(h, 1); }
+        if (rc == MPM_OK) { rc = mpm_last_step_ms(h, ms); }
+        if (rc != MPM_OK) {
+            g_create_error = h->last_error;
+            g_create_msg = h->last_msg;
+        }
+        mpm_destroy(h);
+        return rc;
+    };
+    MpmParams best = *base;
+    best.substeps = substeps;
+    if (best.sort_interval <= 0 || best.sort_interval > substeps) { best.sort_interval = substeps; }
+    float best_ms = 3.4e38f;
+    int32_t tried = 0;
+    const int chunks[4] = {256, 512, 1024, 2048};
+    for (int k = 0; k < kNumShapes; ++k) {
+        for (int c : chunks) {
+            MpmParams p = best;
+            p.block_x = kShapeX[k];
+            p.block_y = kShapeY[k];
+            p.chunk = c;
+            float ms = 0.f;
+            const int32_t rc = time_one(p, &ms);
+            if (rc != MPM_OK) { return rc; }
+            ++tried;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best.block_x = p.block_x;
+                best.block_y = p.block_y;
+                best.chunk = c;
+            }
+        }
+    }
+    const int intervals[3] = {25, 50, 100};
+    for (int iv : intervals) {
+        if (iv > substeps || iv == best.sort_interval) { continue; }
+        MpmParams p = best;
+        p.sort_interval = iv;
+        float ms = 0.f;
+        const int32_t rc = time_one(p, &ms);
+        if (rc != MPM_OK) { return rc; }
+        ++tried;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best.sort_interval = iv;
+        }
+    }
+    out->block_x = best.block_x;
+    out->block_y = best.block_y;
+    out->chunk = best.chunk;
+    out->sort_interval = best.sort_interval;
+    out->ms_per_substep = best_ms / (float)substeps;
+    out->candidates = tried;
     return MPM_OK;
 }
 

@@ -98,6 +98,17 @@ class MpmView(ctypes.Structure):
     ]
 
 
+class MpmTuneResult(ctypes.Structure):
+    _fields_ = [
+        ("block_x", ctypes.c_int32),
+        ("block_y", ctypes.c_int32),
+        ("chunk", ctypes.c_int32),
+        ("sort_interval", ctypes.c_int32),
+        ("ms_per_substep", ctypes.c_float),
+        ("candidates", ctypes.c_int32),
+    ]
+
+
 _P = ctypes.POINTER
 _f32p = _P(ctypes.c_float)
 _i32p = _P(ctypes.c_int32)
@@ -126,6 +137,8 @@ SIGNATURES = {
         [_vp, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
          ctypes.c_float, ctypes.c_uint64, _i32p],
     ),
+    "mpm_autotune": (ctypes.c_int32, [_P(MpmParams), ctypes.c_int32, _f32p, _f32p, _u8p, _i32p, ctypes.c_int32,
+                                      _P(MpmTuneResult)]),
     "mpm_last_error": (ctypes.c_int32, [_vp]),
     "mpm_error_name": (ctypes.c_char_p, [ctypes.c_int32]),
     "mpm_last_message": (ctypes.c_char_p, [_vp]),

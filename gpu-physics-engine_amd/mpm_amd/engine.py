@@ -180,3 +180,23 @@ class Engine:
                                          _fp(tau))
         _check(self.lib, self.h, rc, "mpm_compute_stress")
         return tau
+
+
+def autotune(params, scene, substeps=None):
+    """Run the engine's autotuner (mpm_autotune) on a scene; returns a tuned copy of params and the
+    MpmTuneResult."""
+    lib = abi.load_library()
+    p = params.copy()
+    x = np.ascontiguousarray(scene.x, dtype=np.float32)
+    v = np.ascontiguousarray(scene.v, dtype=np.float32)
+    m = np.ascontiguousarray(scene.material, dtype=np.uint8)
+    sc = np.ascontiguousarray(scene.scene_counts, dtype=np.int32) if p.num_scenes > 1 else None
+    res = abi.MpmTuneResult()
+    rc = lib.mpm_autotune(ctypes.byref(p), x.shape[0], _fp(x), _fp(v), m.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                          None if sc is None else sc.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                          int(substeps or p.substeps), ctypes.byref(res))
+    if rc < 0:
+        msg = lib.mpm_last_message(None)
+        raise MpmError(rc, f"mpm_autotune: {msg.decode() if msg else ''}")
+    p.block_x, p.block_y, p.chunk, p.sort_interval = res.block_x, res.block_y, res.chunk, res.sort_interval
+    return p, res

@@ -159,6 +159,24 @@ int32_t mpm_count(MpmEngine* h, int32_t* n);
 int32_t mpm_hose_pipe(MpmEngine* h, float origin_x, float origin_y, float velocity_x, float velocity_y,
                       float nozzle_width, float spacing, float jitter, uint64_t tick, int32_t* emitted);
 
+/* Autotuner result (no counterpart in the reference, whose launch shape is the compile-time 36 x 256 of
+ * constants.h:18-21). */
+typedef struct MpmTuneResult {
+    int32_t block_x, block_y; /* bin / LDS tile shape                          */
+    int32_t chunk;            /* particles per work item                       */
+    int32_t sort_interval;    /* substeps between re-binnings                  */
+    float ms_per_substep;     /* best measured GPU time per substep            */
+    int32_t candidates;       /* configurations timed                          */
+} MpmTuneResult;
+
+/* Time candidate launch configurations of the binned modes on the caller's scene and return the fastest:
+ * every compiled tile shape x chunk in {256, 512, 1024, 2048} at base->sort_interval, then sort intervals
+ * {base, 25, 50, 100} (<= substeps) for the winner.  Each candidate is a temporary handle that runs one
+ * warm-up tick and one timed tick of `substeps` substeps (HIP events).  Arguments as mpm_set_particles.
+ * No counterpart in the reference. */
+int32_t mpm_autotune(const MpmParams* base, int32_t n, const float* x, const float* v, const uint8_t* material,
+                     const int32_t* scene_counts, int32_t substeps, MpmTuneResult* out);
+
 /* Last hipError_t (or MPM_ERR_*) recorded on the handle (cuda_engine_last_error, cuda-interface.h:41);
  * h may be NULL for errors raised before a handle existed. */
 int32_t mpm_last_error(MpmEngine* h);

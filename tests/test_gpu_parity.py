@@ -289,3 +289,20 @@ def test_time_kernels_advances_like_step(oracle_mod):
     assert rel(a["x"], b["x"]) < 1e-6
     assert t["g2p2g"][1] == 8 and t["binning"][1] == 2 and t["grid_update"][1] == 10
     assert all(ms >= 0 for ms, _ in t.values())
+
+
+def test_autotune_picks_a_valid_configuration(oracle_mod):
+    """mpm_autotune times every compiled tile shape x chunk on the caller's scene and returns one of them;
+    the tuned engine still matches the FP64 oracle."""
+    from mpm_amd.engine import autotune
+    p = abi.preset(abi.PRESET_MIXED1024)
+    p.substeps = 20
+    p.sort_interval = 10
+    sc = scenes.mixed(n=65536)
+    tuned, res = autotune(p, sc)
+    assert (res.block_x, res.block_y) in [(8, 8), (16, 8), (16, 16), (32, 16)]
+    assert res.chunk in (256, 512, 1024, 2048) and res.candidates >= 16
+    assert res.ms_per_substep > 0
+    out = run_gpu(tuned, sc.x, sc.v, sc.material)
+    s, _ = run_oracle(oracle_mod, tuned, sc.x, sc.v, sc.material, tuned.substeps)
+    assert rel(out["x"], s.x) < 1e-6
