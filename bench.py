@@ -46,7 +46,14 @@ WORKLOADS = {
     "batch64x256": (abi.PRESET_BATCH64X256, lambda scale: scenes.batched(scale=scale),
                     "batch64x256: 64 scenes of 256x256 cells, particle counts log-uniform in [4096, 524288], "
                     "mixed materials, dt=2e-5, 200 substeps per step"),
+    "learned1024": (abi.PRESET_LEARNED1024, lambda scale: scenes.for_preset(abi.PRESET_LEARNED1024, scale),
+                    "learned1024: 4,194,304 jelly + snow particles on a 1024x1024-cell grid, BF16 MLP closure, "
+                    "dt=2e-5, 100 substeps per step"),
+    "learned16x2048": (abi.PRESET_LEARNED16X2048, lambda scale: scenes.for_preset(abi.PRESET_LEARNED16X2048, scale),
+                       "learned16x2048: 16 scenes of 2048x2048 cells, 1,048,576-16,777,216 jelly + snow particles "
+                       "each, BF16 MLP closure, dt=1e-5, 100 substeps per step"),
 }
+CLOSURE_WEIGHTS = os.path.join(ROOT, "gpu-physics-engine_amd", "mpm_amd", "data", "closure_learned1024.npy")
 
 
 def traffic_from_profile(p, workload):
@@ -150,6 +157,9 @@ def main():
             f"{res.ms_per_substep * 1e3:.1f} us/substep")
     eng = Engine(p)
     eng.set_scene(scene)
+    if p.closure == abi.CLOSURE_LEARNED:
+        eng.set_closure(np.load(CLOSURE_WEIGHTS))
+        log(f"learned closure loaded from {os.path.relpath(CLOSURE_WEIGHTS, ROOT)}")
     log("engine created, state resident in HBM")
     for _ in range(args.warmup):
         eng.step(1)
@@ -215,10 +225,43 @@ def main():
         "kernels": kernels,
         "state_finite": finite,
     }
+    if p.closure == abi.CLOSURE_LEARNED:
+        result["vs_plain"] = compare_plain(p, scene, args, value)
+        result["roofline"]["mfma_tflops"] = scene.n * 2 * 64 * 64 / (ms_dom / n_dom * 1e-3) / 1e12 if n_dom else None
     if not args.no_cpu_baseline:
         log("cpu baseline (FP64 oracle) ...")
         result["cpu_baseline"] = cpu_baseline(p, scene)
     print(json.dumps(result), flush=True)
+
+
+def compare_plain(p, scene, args, learned_value):
+    """The same scene with the analytic closure: throughput over the same number of ticks, and the relative
+    position error of the learned closure after one tick from the initial state."""
+    pa = p.copy()
+    pa.closure = abi.CLOSURE_ANALYTIC
+    with Engine(pa) as e:
+        e.set_scene(scene)
+        e.step(1)
+        e.sync()
+        x_plain = e.readback()["x"].astype(np.float64)
+        for _ in range(args.warmup):
+            e.step(1)
+        e.sync()
+        t0 = time.perf_counter()
+        e.step(args.steps)
+        e.sync()
+        plain_value = scene.n * pa.substeps * args.steps / (time.perf_counter() - t0)
+    with Engine(p) as e:
+        e.set_scene(scene)
+        e.set_closure(np.load(CLOSURE_WEIGHTS))
+        e.step(1)
+        e.sync()
+        x_learned = e.readback()["x"].astype(np.float64)
+    err = float(np.linalg.norm(x_learned - x_plain) / np.linalg.norm(x_plain))
+    log(f"plain (analytic) on the same scene: {plain_value:.4g} particle-substeps/s; learned / plain "
+        f"{learned_value / plain_value:.3f}; relative position error after {p.substeps} substeps {err:.2e}")
+    return {"plain_value": plain_value, "learned_over_plain": learned_value / plain_value,
+            "rel_position_error_vs_plain": err, "substeps_compared": p.substeps}
 
 
 if __name__ == "__main__":

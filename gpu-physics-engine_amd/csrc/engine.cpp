@@ -48,6 +48,9 @@ struct MpmEngine {
     int32_t pending_slot = -1;
     int32_t last_count = 0;
     int shape = 2;
+    float* closure_w = nullptr;   // device float[4675]
+    void* closure_w2 = nullptr;   // device W2 in the MFMA operand layout (8 KiB)
+    bool closure_loaded = false;
     bool has_ticked = false;
     int32_t last_error = 0;
     std::string last_msg;
@@ -217,6 +220,8 @@ int32_t allocate(MpmEngine* h)
         ss.items = c.take<Item>((size_t)ss.max_items);
         ss.num_items = c.take<int32_t>(1);
         h->count = c.take<int32_t>(2);
+        h->closure_w = c.take<float>(4675);
+        h->closure_w2 = c.take<uint8_t>(8 * 64 * 16);
         for (int k = 0; k < 3; ++k) {
             h->out_pos[k] = c.take<float>(2 * cap);
             h->out_vel[k] = c.take<float>(2 * cap);
@@ -229,6 +234,8 @@ int32_t allocate(MpmEngine* h)
     }
     // Zero the whole slab once (grid accumulators must start at zero; counts at zero).
     HIP_TRY(h, hipMemsetAsync(h->slab, 0, h->slab_bytes, h->stream));
+    h->dp.cl.w = h->closure_w;
+    h->dp.cl.w2_mfma = reinterpret_cast<const bf16x8*>(h->closure_w2);
     HIP_TRY(h, hipHostMalloc((void**)&h->staged, 3 * sizeof(OutView), hipHostMallocDefault));
     for (int k = 0; k < 3; ++k) {
         h->staged[k].pos = h->out_pos[k];
@@ -497,8 +504,8 @@ int32_t mpm_create(const MpmParams* params, MpmEngine** out)
     MpmParams p = *params;
     const std::string bad = validate(p);
     if (!bad.empty()) { return fail(nullptr, MPM_ERR_INVALID_ARG, 0, bad); }
-    if (p.closure == MPM_CLOSURE_LEARNED) {
-        return fail(nullptr, MPM_ERR_NOT_BUILT, 0, "learned closure is not compiled into this library yet");
+    if (p.closure == MPM_CLOSURE_LEARNED && p.mode == MPM_MODE_REFERENCE) {
+        return fail(nullptr, MPM_ERR_NOT_BUILT, 0, "the learned closure runs in the binned modes only");
     }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
@@ -606,6 +613,9 @@ int32_t mpm_step(MpmEngine* h, int32_t n_ticks)
 {
     if (!h) { return MPM_ERR_INVALID_ARG; }
     if (n_ticks < 0) { return fail(h, MPM_ERR_INVALID_ARG, 0, "n_ticks < 0"); }
+    if (h->params.closure == MPM_CLOSURE_LEARNED && !h->closure_loaded) {
+        return fail(h, MPM_ERR_STATE, 0, "learned closure selected but no weights loaded (mpm_set_closure)");
+    }
     for (int32_t k = 0; k < n_ticks; ++k) {
         const int32_t slot = h->write_slot;
         HIP_TRY(h, set_output_view_async(h->stream, &h->staged[slot]));
@@ -757,6 +767,20 @@ int32_t mpm_hose_pipe(MpmEngine* h, float origin_x, float origin_y, float veloci
     return MPM_OK;
 }
 
+int32_t mpm_set_closure(MpmEngine* h, const float* weights)
+{
+    if (!h || !weights) { return MPM_ERR_INVALID_ARG; }
+    for (int i = 0; i < 4675; ++i) {
+        if (!std::isfinite(weights[i])) { return fail(h, MPM_ERR_INVALID_ARG, 0, "closure weights not finite"); }
+    }
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    HIP_TRY(h, hipMemcpy(h->closure_w, weights, 4675 * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(h, launch_pack_w2(h->stream, h->closure_w, h->closure_w2));
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    h->closure_loaded = true;
+    return MPM_OK;
+}
+
 int32_t mpm_autotune(const MpmParams* base, int32_t n, const float* x, const float* v, const uint8_t* material,
                      const int32_t* scene_counts, int32_t substeps, MpmTuneResult* out)
 {
@@ -867,6 +891,9 @@ int32_t mpm_debug_bin(MpmEngine* h, uint32_t* keys, int32_t* ids, int32_t* n)
 int32_t mpm_time_kernels(MpmEngine* h, int32_t substeps, float* ms, int32_t* launches)
 {
     if (!h || !ms || !launches || substeps < 1) { return MPM_ERR_INVALID_ARG; }
+    if (h->params.closure == MPM_CLOSURE_LEARNED && !h->closure_loaded) {
+        return fail(h, MPM_ERR_STATE, 0, "learned closure selected but no weights loaded (mpm_set_closure)");
+    }
     HIP_TRY(h, hipStreamSynchronize(h->stream));
     LaunchTimer tm;
     tm.st = h->stream;

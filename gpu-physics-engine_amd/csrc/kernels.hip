@@ -12,6 +12,7 @@
 // plus the stable counting/radix binning that the reference MPM does not have (the reference's only
 // sort, PBF.cu:738-834, ranks with atomicAdd and is order-nondeterministic; this one is stable).
 #include "kernels.h"
+#include "closure.h"
 
 #include <hip/hip_runtime.h>
 
@@ -723,7 +724,7 @@ struct CeilLog2<1> {
 // is fixed by the particle order.  A particle that drifted out of the tile scatters to HBM atomically.
 template <int BX, int BY>
 __device__ __forceinline__ void scatter(const DevParams& p, const Grid& g, size_t gb, int ox, int oy, float4* wtile,
-                                        float px, float py, float vx, float vy, const Mat2& Q)
+                                        float px, float py, float vx, float vy, const Mat2& Q, bool active = true)
 {
     using T = TileShape<BX, BY>;
     constexpr int kBits = CeilLog2<T::N>::value;
@@ -734,7 +735,7 @@ __device__ __forceinline__ void scatter(const DevParams& p, const Grid& g, size_
     bx = clampi(bx, 0, p.W - 3);
     by = clampi(by, 0, p.H - 3);
     const int lx = bx - ox, ly = by - oy;
-    const bool inside = (unsigned)lx <= (unsigned)(T::TX - 3) && (unsigned)ly <= (unsigned)(T::TY - 3);
+    const bool inside = active && (unsigned)lx <= (unsigned)(T::TX - 3) && (unsigned)ly <= (unsigned)(T::TY - 3);
     // contribution to stencil node (a, b): w_ab (m v + Q d_ab, m) with d_ab = ((a - fx) dx, (b - fy) dx),
     // i.e. w_ab (A0 + a dx Q.col0 + b dx Q.col1, m), A0 = m v - dx Q f, built incrementally over a and b.
     const float qax = p.dx * Q.a00, qay = p.dx * Q.a10, qbx = p.dx * Q.a01, qby = p.dx * Q.a11;
@@ -758,7 +759,7 @@ __device__ __forceinline__ void scatter(const DevParams& p, const Grid& g, size_
         tax += qax;
         tay += qay;
     }
-    if (!inside) {
+    if (active && !inside) {
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
 #pragma unroll
@@ -905,6 +906,106 @@ __global__ __launch_bounds__(kThreads) void k_g2p2g(DevParams p, Store s, Grid g
     flush_acc_tiles<BX, BY>(p, g, gb, ox, oy, acc);
 }
 
+// Affine momentum with the learned closure: jelly and snow particles inside the trained range take
+// tau = mu * MLP(features); fluid and out-of-range particles the analytic stress.  Wave-uniform: the
+// network runs once per wave when any lane needs it (all 64 lanes must be active).
+__device__ __forceinline__ Mat2 affine_q_learned(const DevParams& p, const ClosureLds* cl, int mat, const Mat2& C,
+                                                 const Mat2& F, float J)
+{
+    float x[4] = {0.f, 0.f, 0.f, 0.f};
+    bool use = false;
+    if (mat != MPM_MAT_FLUID) { use = closure_features(F, mat == MPM_MAT_SNOW ? J : 1.f, x); }
+    float y[3] = {0.f, 0.f, 0.f};
+    if (__ballot(use) != 0ull) { closure_eval_wave(cl, x, y); }
+    if (use) {
+        const float D = p.dt * p.vol * 4.f * p.inv_dx * p.inv_dx;
+        const float t00 = p.mu * y[0], t01 = p.mu * y[1], t11 = p.mu * y[2];
+        return {p.mass * C.a00 - D * t00, p.mass * C.a01 - D * t01, p.mass * C.a10 - D * t01,
+                p.mass * C.a11 - D * t11};
+    }
+    return affine_q(p, mat, C, F, J);
+}
+
+// k_p2g_tiled with the learned closure (uniform trip count so every wave runs the network with 64 lanes).
+template <int BX, int BY>
+__global__ __launch_bounds__(kThreads) void k_p2g_tiled_learned(DevParams p, Store s, Grid g, const Item* items,
+                                                                const int32_t* num_items, const int32_t* count)
+{
+    using T = TileShape<BX, BY>;
+    __shared__ float4 acc[T::N * (kThreads / 64)];
+    __shared__ ClosureLds cl;
+    bool ok;
+    const int it = xcd_item(num_items[0], ok);
+    if (!ok) { return; }
+    Item item = items[it];
+    item.end = min(item.end, count[0]);
+    if (item.end <= item.begin) { return; }
+    int scene, ox, oy;
+    item_origin(p, item.key, BX, BY, scene, ox, oy);
+    const size_t gb = (size_t)scene * (size_t)p.G;
+    zero_acc_tiles<BX, BY>(acc);
+    closure_stage(p.cl, &cl);
+    __syncthreads();
+    float4* wtile = acc + (threadIdx.x >> 6) * T::N;
+    for (int base = item.begin; base < item.end; base += kThreads) {
+        const int j = base + threadIdx.x;
+        const bool valid = j < item.end;
+        const uint32_t o = (uint32_t)(valid ? j : item.end - 1) * 4u;
+        const int mat = ldo(s.mat, o >> 2);
+        const Mat2 Q = affine_q_learned(p, &cl, mat, ld_C(s, o), ld_F(s, o), ldo(s.J, o));
+        scatter<BX, BY>(p, g, gb, ox, oy, wtile, ldo(s.x, o), ldo(s.y, o), ldo(s.vx, o), ldo(s.vy, o), Q, valid);
+    }
+    __syncthreads();
+    flush_acc_tiles<BX, BY>(p, g, gb, ox, oy, acc);
+}
+
+// k_g2p2g with the learned closure: the P2G stress of every wave comes from one BF16 MFMA evaluation of
+// the network for its 64 particles.
+template <int BX, int BY>
+__global__ __launch_bounds__(kThreads) void k_g2p2g_learned(DevParams p, Store s, Grid g, const Item* items,
+                                                            const int32_t* num_items, const int32_t* count)
+{
+    using T = TileShape<BX, BY>;
+    __shared__ float2 sv[T::N];
+    __shared__ float4 acc[T::N * (kThreads / 64)];
+    __shared__ ClosureLds cl;
+    bool ok;
+    const int it = xcd_item(num_items[0], ok);
+    if (!ok) { return; }
+    Item item = items[it];
+    item.end = min(item.end, count[0]);
+    if (item.end <= item.begin) { return; }
+    int scene, ox, oy;
+    item_origin(p, item.key, BX, BY, scene, ox, oy);
+    const size_t gb = (size_t)scene * (size_t)p.G;
+    load_vel_tile<BX, BY>(p, g, gb, ox, oy, sv);
+    zero_acc_tiles<BX, BY>(acc);
+    closure_stage(p.cl, &cl);
+    __syncthreads();
+    float4* wtile = acc + (threadIdx.x >> 6) * T::N;
+    for (int base = item.begin; base < item.end; base += kThreads) {
+        const int j = base + threadIdx.x;
+        const bool valid = j < item.end;
+        const uint32_t o = (uint32_t)(valid ? j : item.end - 1) * 4u;
+        float px = ldo(s.x, o), py = ldo(s.y, o), vx, vy;
+        const int mat = ldo(s.mat, o >> 2);
+        Mat2 C, B, F = ld_F(s, o);
+        float J = ldo(s.J, o);
+        gather<BX, BY>(p, g, gb, ox, oy, sv, px, py, vx, vy, B);
+        g2p_update(p, mat, px, py, vx, vy, B, C, F, J);
+        if (valid) {
+            sto(s.x, o, px);
+            sto(s.y, o, py);
+            st_F(s, o, F);
+            sto(s.J, o, J);
+        }
+        const Mat2 Q = affine_q_learned(p, &cl, mat, C, F, J);
+        scatter<BX, BY>(p, g, gb, ox, oy, wtile, px, py, vx, vy, Q, valid);
+    }
+    __syncthreads();
+    flush_acc_tiles<BX, BY>(p, g, gb, ox, oy, acc);
+}
+
 int shape_index(int bx, int by)
 {
     for (int k = 0; k < kNumShapes; ++k) {
@@ -932,6 +1033,10 @@ hipError_t launch_p2g_tiled(hipStream_t st, int shape, const DevParams& p, const
                             const SortScratch& ss, const int32_t* count)
 {
     const unsigned nb = item_blocks(ss);
+    if (p.closure == MPM_CLOSURE_LEARNED) {
+        MPM_DISPATCH_SHAPE(k_p2g_tiled_learned)
+        return hipGetLastError();
+    }
     MPM_DISPATCH_SHAPE(k_p2g_tiled)
     return hipGetLastError();
 }
@@ -946,6 +1051,10 @@ hipError_t launch_g2p2g(hipStream_t st, int shape, const DevParams& p, const Sto
                         const SortScratch& ss, const int32_t* count)
 {
     const unsigned nb = item_blocks(ss);
+    if (p.closure == MPM_CLOSURE_LEARNED) {
+        MPM_DISPATCH_SHAPE(k_g2p2g_learned)
+        return hipGetLastError();
+    }
     MPM_DISPATCH_SHAPE(k_g2p2g)
     return hipGetLastError();
 }

@@ -21,6 +21,14 @@ constexpr int kRadixTile = kThreads * kRadixItems;
 constexpr int kMargin = 2;           // nodes of drift a binned particle may take before leaving its tile
 constexpr int kMaxScenes = 4096;
 
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+
+// Device copy of the learned closure's parameters (see closure.h).
+struct ClosureDev {
+    const float* w;         // float[4675]
+    const bf16x8* w2_mfma;  // W2 packed as the MFMA A operand
+};
+
 // Kernel-argument copy of MpmParams plus derived quantities (passed by value, lives in SGPRs/kernarg).
 struct DevParams {
     int W, H, halo, wall;
@@ -35,6 +43,7 @@ struct DevParams {
     float mu, lambda, theta_c, theta_s, hardening;
     float j_min, j_max;
     int closure;
+    ClosureDev cl;
 };
 
 // SoA particle store.  Every component array holds `capacity` entries, 256-byte aligned.

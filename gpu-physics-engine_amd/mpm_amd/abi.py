@@ -139,6 +139,13 @@ SIGNATURES = {
     ),
     "mpm_autotune": (ctypes.c_int32, [_P(MpmParams), ctypes.c_int32, _f32p, _f32p, _u8p, _i32p, ctypes.c_int32,
                                       _P(MpmTuneResult)]),
+    "mpm_closure_samples": (ctypes.c_int32, [_P(MpmParams), ctypes.c_int32, _f32p, _f32p, _u8p, _f32p, _f32p, _i32p]),
+    "mpm_closure_train": (ctypes.c_int32, [ctypes.c_int32, _f32p, _f32p, ctypes.c_int32, ctypes.c_int32, ctypes.c_float,
+                                           ctypes.c_uint64, _f32p, _f32p]),
+    "mpm_closure_grad": (ctypes.c_int32, [ctypes.c_int32, _f32p, _f32p, _f32p, _f32p, _f32p]),
+    "mpm_closure_eval": (ctypes.c_int32, [ctypes.c_int32, _f32p, _f32p, _f32p]),
+    "mpm_closure_message": (ctypes.c_char_p, []),
+    "mpm_set_closure": (ctypes.c_int32, [_vp, _f32p]),
     "mpm_last_error": (ctypes.c_int32, [_vp]),
     "mpm_error_name": (ctypes.c_char_p, [ctypes.c_int32]),
     "mpm_last_message": (ctypes.c_char_p, [_vp]),
@@ -147,6 +154,8 @@ SIGNATURES = {
     "mpm_time_kernels": (ctypes.c_int32, [_vp, ctypes.c_int32, _f32p, _i32p]),
     "mpm_compute_stress": (ctypes.c_int32, [_vp, ctypes.c_int32, _u8p, _f32p, _f32p, _f32p]),
 }
+
+CLOSURE_NUM_WEIGHTS = 4675
 
 KT_BINNING = 0
 KT_P2G = 1

@@ -143,6 +143,13 @@ class Engine:
         shape = (p.num_scenes, p.grid_h, p.grid_w)
         return mass.reshape(shape), vx.reshape(shape), vy.reshape(shape)
 
+    def set_closure(self, weights):
+        """Load learned-closure parameters (float[4675]) into the handle."""
+        w = np.ascontiguousarray(weights, dtype=np.float32).reshape(-1)
+        if w.size != abi.CLOSURE_NUM_WEIGHTS:
+            raise ValueError(f"closure weights must have {abi.CLOSURE_NUM_WEIGHTS} entries")
+        _check(self.lib, self.h, self.lib.mpm_set_closure(self.h, _fp(w)), "mpm_set_closure")
+
     def read_view(self):
         view = abi.MpmView()
         _check(self.lib, self.h, self.lib.mpm_read_view(self.h, ctypes.byref(view)), "mpm_read_view")

@@ -159,6 +159,34 @@ int32_t mpm_count(MpmEngine* h, int32_t* n);
 int32_t mpm_hose_pipe(MpmEngine* h, float origin_x, float origin_y, float velocity_x, float velocity_y,
                       float nozzle_width, float spacing, float jitter, uint64_t tick, int32_t* emitted);
 
+/* ---- learned constitutive closure (no counterpart in the reference, whose only stress is the Tait fluid,
+ * MPM.cu:68-80) ----
+ * A 4-64-64-3 SiLU network maps (s1, s2, theta, Jh) — singular values of the elastic F (s1 >= s2), angle of
+ * U, and Jp for snow / 1 for jelly, all normalised — to tau / mu (tau00, tau01, tau11).  Parameters are a
+ * float[MPM_CLOSURE_NUM_WEIGHTS] array: W1[64][4] | b1[64] | W2[64][64] | b2[64] | W3[3][64] | b3[3].
+ * Inference runs as BF16 MFMA tiles with FP32 accumulation inside the binned G2P/P2G kernels; jelly and snow
+ * particles outside s in [0.6, 1.6], Jh in [0.7, 1.3] and all fluid particles use the analytic stress. */
+#define MPM_CLOSURE_NUM_WEIGHTS 4675
+
+/* Training pairs from particle states: features float[4n] and targets float[3n] (analytic tau / mu) computed
+ * on the device; valid[i] = 1 for jelly / snow particles inside the trained range. */
+int32_t mpm_closure_samples(const MpmParams* params, int32_t n, const float* F, const float* J,
+                            const uint8_t* material, float* features, float* targets, int32_t* valid);
+/* Train on the GPU (hand-written forward, backward and Adam kernels, FP32): `steps` minibatches of `batch`
+ * pairs drawn by a seeded hash from (features, targets), mean squared error.  weights: initial values in,
+ * trained values out.  *final_loss: loss of the result on one more minibatch. */
+int32_t mpm_closure_train(int32_t n, const float* features, const float* targets, int32_t steps, int32_t batch,
+                          float lr, uint64_t seed, float* weights, float* final_loss);
+/* One forward + backward over all n pairs on the GPU: mean loss and its gradient float[MPM_CLOSURE_NUM_WEIGHTS]. */
+int32_t mpm_closure_grad(int32_t n, const float* features, const float* targets, const float* weights, float* grad,
+                         float* loss);
+/* Network outputs float[3n] of n feature vectors through the BF16 MFMA inference path of the engine. */
+int32_t mpm_closure_eval(int32_t n, const float* features, const float* weights, float* out);
+/* Message of the last failed closure call. */
+const char* mpm_closure_message(void);
+/* Load trained parameters into a handle; required before mpm_step when params.closure is MPM_CLOSURE_LEARNED. */
+int32_t mpm_set_closure(MpmEngine* h, const float* weights);
+
 /* Autotuner result (no counterpart in the reference, whose launch shape is the compile-time 36 x 256 of
  * constants.h:18-21). */
 typedef struct MpmTuneResult {

@@ -1,0 +1,168 @@
+"""GPU tests of the learned constitutive closure: gradients against the FP64 oracle, the Adam step's closed
+form, the BF16 MFMA inference against a BF16-emulating reference, feature extraction, and what a trained
+network gains over an untrained one (stress error and trajectory error against the analytic model and the
+FP64 oracle)."""
+import numpy as np
+import pytest
+
+from mpm_amd import abi, closure, scenes
+from mpm_amd.engine import Engine
+
+pytestmark = pytest.mark.gpu
+
+
+def bf16(a):
+    """Round float32 to bfloat16 (round to nearest even), returned as float32."""
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32)
+
+
+def silu(z):
+    return z / (1.0 + np.exp(-z))
+
+
+def forward_bf16(X, w):
+    """Restatement of the device inference: FP32 layer 1, BF16 operands of layer 2 with FP32 accumulation,
+    FP32 layer 3."""
+    P = closure.unflatten(w)
+    h1 = silu(X.astype(np.float64) @ P["W1"].T + P["b1"])
+    z2 = bf16(h1).astype(np.float64) @ bf16(P["W2"]).astype(np.float64).T + P["b2"]
+    return silu(z2) @ P["W3"].T + P["b3"]
+
+
+def test_gradient_matches_fp64_oracle(oracle_mod):
+    rng = np.random.default_rng(3)
+    w = closure.init_weights(5)
+    w += rng.normal(0, 0.05, w.shape).astype(np.float32)
+    X = rng.normal(0, 1, (512, 4)).astype(np.float32)
+    T = rng.normal(0, 1, (512, 3)).astype(np.float32)
+    loss, g = closure.grad(X, T, w)
+    net = oracle_mod.MLP(closure.unflatten(w))
+    ref_loss, ref_g = net.loss_grad(X.astype(np.float64), T.astype(np.float64))
+    ref = np.concatenate([ref_g[k].reshape(-1) for k, _ in closure.SHAPES])
+    assert loss == pytest.approx(ref_loss, rel=1e-5)
+    assert np.max(np.abs(g - ref)) < 1e-5 * max(1.0, np.abs(ref).max())
+
+
+def test_first_adam_step_and_training_reduces_loss():
+    rng = np.random.default_rng(4)
+    X = rng.uniform(-1, 1, (4096, 4)).astype(np.float32)
+    T = np.stack([np.sin(X[:, 0]) + X[:, 3], X[:, 1] * X[:, 2], np.cos(X[:, 2])], 1).astype(np.float32)
+    w0 = closure.init_weights(1)
+    w1, _ = closure.train(X, T, steps=1, batch=4096, lr=1e-3, seed=9, weights=w0)
+    d = np.abs(w1.astype(np.float64) - w0)
+    # first Adam step from zero moments moves every parameter by lr * |g| / (|g| + eps): at most lr, and lr
+    # up to rounding wherever |g| >> eps
+    moved = d > 0
+    assert moved.mean() > 0.9
+    assert d.max() <= 1e-3 * (1 + 1e-3)
+    assert np.mean(np.abs(d[moved] - 1e-3) < 1e-5) > 0.95
+    loss0, _ = closure.grad(X, T, w0)
+    w, last = closure.train(X, T, steps=600, batch=2048, lr=3e-3, seed=9, weights=w0)
+    loss1, _ = closure.grad(X, T, w)
+    assert loss1 < 0.05 * loss0
+
+
+def test_mfma_inference_matches_bf16_reference(oracle_mod):
+    """The wave-level BF16 MFMA path (checked lane layout) against a BF16-emulating numpy restatement and the
+    FP64 oracle forward."""
+    rng = np.random.default_rng(6)
+    w = closure.init_weights(2)
+    w += rng.normal(0, 0.1, w.shape).astype(np.float32)
+    X = rng.uniform(-1.2, 1.2, (1000, 4)).astype(np.float32)  # not a multiple of 64: partial last wave
+    y = closure.evaluate(X, w)
+    ref = forward_bf16(X, w)
+    scale = np.abs(ref).max()
+    assert np.max(np.abs(y - ref)) < 2e-3 * scale
+    exact = oracle_mod.MLP(closure.unflatten(w)).forward(X.astype(np.float64))
+    assert np.max(np.abs(y - exact)) < 5e-2 * scale
+
+
+def test_features_and_targets(oracle_mod):
+    p = abi.preset(abi.PRESET_LEARNED1024)
+    th = 0.4
+    R = np.array([[np.cos(th), -np.sin(th)], [np.sin(th), np.cos(th)]])
+    F = np.stack([np.diag([1.2, 0.9]).reshape(4), (R @ np.diag([0.9, 1.3]) @ R.T).reshape(4),
+                  np.diag([2.0, 1.0]).reshape(4), np.diag([1.1, -1.0]).reshape(4)]).astype(np.float32)
+    J = np.array([1.0, 1.1, 1.0, 1.0], np.float32)
+    mat = np.array([abi.MAT_JELLY, abi.MAT_SNOW, abi.MAT_JELLY, abi.MAT_JELLY], np.uint8)
+    X, T, V = closure.samples(p, F, J, mat)
+    assert V.tolist() == [True, True, False, False]
+    assert np.allclose(X[0], [(1.2 - 1.1) * 2, (0.9 - 1.1) * 2, 0.0, 0.0], atol=1e-5)
+    # second: s = (1.3, 0.9) with U rotated by th + 90 deg (1.3 is the second axis) -> th - 90 deg mod 180
+    ang = ((th + np.pi / 2 + np.pi / 2) % np.pi) - np.pi / 2
+    assert np.allclose(X[1], [(1.3 - 1.1) * 2, (0.9 - 1.1) * 2, ang / (np.pi / 2), 0.1 / 0.3], atol=1e-4)
+    tau = oracle_mod.tau(p, mat, F.astype(np.float64), J.astype(np.float64))
+    assert np.allclose(T[:2], tau[:2][:, [0, 1, 3]] / np.float64(np.float32(p.mu)), rtol=1e-4, atol=1e-5)
+
+
+@pytest.fixture(scope="module")
+def trained():
+    """Closure trained on the GPU from a harvest of the plain solver plus random deformations."""
+    p = abi.preset(abi.PRESET_LEARNED1024)
+    p.substeps = 50
+    sc = scenes.for_preset(abi.PRESET_LEARNED1024, scale=1 / 16)
+    harvested = closure.harvest(p, sc, ticks=4)
+    X, T = closure.dataset(p, harvested, n_random=400000, seed=11)
+    w, loss = closure.train(X, T, steps=3000, batch=8192, lr=2e-3, seed=3)
+    return p, w, loss, X, T
+
+
+def test_trained_network_gain_in_stress(trained):
+    """Held-out stress error of the trained network versus the same network untrained."""
+    p, w, loss, _, _ = trained
+    F, J, M = closure.random_deformations(20000, seed=99)
+    Xh, Th, V = closure.samples(p, F, J, M)
+    Xh, Th = Xh[V], Th[V]
+    err_trained = np.sqrt(np.mean((closure.evaluate(Xh, w) - Th) ** 2))
+    err_untrained = np.sqrt(np.mean((closure.evaluate(Xh, closure.init_weights(3)) - Th) ** 2))
+    scale = np.sqrt(np.mean(Th ** 2))
+    assert err_trained < 0.05 * scale
+    assert err_trained < 0.1 * err_untrained
+
+
+def test_learned_trajectory_error(trained, oracle_mod):
+    """100 substeps of the jelly + snow scene: the learned closure stays within 1e-3 relative position error
+    of the analytic engine and of the FP64 oracle; an untrained network does not."""
+    p, w, _, _, _ = trained
+    p = p.copy()
+    p.substeps = 100
+    sc = scenes.for_preset(abi.PRESET_LEARNED1024, scale=1 / 64)
+    p.capacity = sc.n
+    outs = {}
+    for name, weights in (("trained", w), ("untrained", closure.init_weights(3))):
+        with Engine(p) as e:
+            e.set_scene(sc)
+            e.set_closure(weights)
+            e.step(1)
+            e.sync()
+            outs[name] = e.readback()
+    pa = p.copy()
+    pa.closure = abi.CLOSURE_ANALYTIC
+    with Engine(pa) as e:
+        e.set_scene(sc)
+        e.step(1)
+        e.sync()
+        plain = e.readback()
+    s = oracle_mod.State(sc.x, sc.v, material=sc.material)
+    oracle_mod.run(pa, s, p.substeps, threads=8)
+
+    def rel(a, b):
+        return np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b)
+    assert rel(plain["x"], s.x) < 1e-5
+    assert rel(outs["trained"]["x"], plain["x"]) < 1e-3
+    assert rel(outs["trained"]["x"], s.x) < 1e-3
+    assert rel(outs["untrained"]["x"], plain["x"]) > 3 * rel(outs["trained"]["x"], plain["x"])
+
+
+def test_learned_closure_requires_weights():
+    p = abi.preset(abi.PRESET_LEARNED1024)
+    sc = scenes.for_preset(abi.PRESET_LEARNED1024, scale=1 / 256)
+    p.capacity = sc.n
+    with Engine(p) as e:
+        e.set_scene(sc)
+        with pytest.raises(Exception) as ei:
+            e.step(1)
+        assert ei.value.code == abi.MPM_ERR_STATE
