@@ -32,13 +32,33 @@ __device__ __forceinline__ float silu(float z)
     return z * __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(-1.44269504f * z));
 }
 
+// atan2(y, x): atan on [0, 1] from the degree-9 odd polynomial of Abramowitz & Stegun 4.4.47 (|error| <=
+// 1e-5 rad) plus octant reduction.  Training and inference use this same function, so the network's angle
+// input is consistent.
+__device__ __forceinline__ float fast_atan2(float y, float x)
+{
+    const float ax = fabsf(x), ay = fabsf(y);
+    const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
+    const float t = mx > 0.f ? mn * __builtin_amdgcn_rcpf(mx) : 0.f;
+    const float t2 = t * t;
+    float r = 0.0208351f;
+    r = fmaf(r, t2, -0.0851330f);
+    r = fmaf(r, t2, 0.1801410f);
+    r = fmaf(r, t2, -0.3302995f);
+    r = fmaf(r, t2, 0.9998660f);
+    r *= t;
+    if (ay > ax) { r = 1.57079632679f - r; }
+    if (x < 0.f) { r = 3.14159265359f - r; }
+    return y < 0.f ? -r : r;
+}
+
 // Closure features of one particle; returns false when outside the trained range (then x is unspecified).
 __device__ __forceinline__ bool closure_features(const Mat2& F, float jh, float x[4])
 {
     float uc, us, s0, s1, vc, vs;
     svd2(F, uc, us, s0, s1, vc, vs);
     // canonical order s0 >= s1: swapping the singular values rotates U by 90 degrees (tau is unchanged)
-    float th = atan2f(us, uc);
+    float th = fast_atan2(us, uc);
     if (s1 > s0) {
         const float t = s0;
         s0 = s1;
@@ -55,107 +75,124 @@ __device__ __forceinline__ bool closure_features(const Mat2& F, float jh, float 
     return s0 >= kSigLo && s0 <= kSigHi && s1 >= kSigLo && s1 <= kSigHi && jh >= kJhLo && jh <= kJhHi;
 }
 
-// LDS staging of the small layers (per work group): W1 rows as float4, b1, b2, W3 columns as float4
-// (W3[0][u], W3[1][u], W3[2][u], 0), b3.
-struct ClosureLds {
-    float4 w1[kHid];
-    float b1[kHid];
-    float b2[kHid];
-    float4 w3[kHid];
-    float b3[4];
-    bf16x8 w2[8 * 64];
-};
-
-__device__ __forceinline__ void closure_stage(const ClosureDev& c, ClosureLds* s)
+// Hidden unit fed as K index kk = 8 h + j of K-step ks: the C register r = 8 (ks % 2) + j of M-tile ks / 2
+// of the previous layer, whose unit is 32 (ks/2) + 8 (r/4) + 4 h + r%4.  With this order a layer's FP32
+// output (C layout) is the next layer's B operand lane by lane, so layers chain without lane exchanges.
+__host__ __device__ constexpr int chain_unit(int ks, int h, int j)
 {
-    for (int u = threadIdx.x; u < kHid; u += blockDim.x) {
-        s->w1[u] = make_float4(c.w[kOffW1 + 4 * u], c.w[kOffW1 + 4 * u + 1], c.w[kOffW1 + 4 * u + 2],
-                               c.w[kOffW1 + 4 * u + 3]);
-        s->b1[u] = c.w[kOffB1 + u];
-        s->b2[u] = c.w[kOffB2 + u];
-        s->w3[u] = make_float4(c.w[kOffW3 + u], c.w[kOffW3 + kHid + u], c.w[kOffW3 + 2 * kHid + u], 0.f);
-    }
-    if (threadIdx.x < 4) { s->b3[threadIdx.x] = threadIdx.x < 3 ? c.w[kOffB3 + threadIdx.x] : 0.f; }
-    for (int i = threadIdx.x; i < 8 * 64; i += blockDim.x) { s->w2[i] = c.w2_mfma[i]; }
+    return 32 * (ks / 2) + 16 * (ks % 2) + 8 * (j / 4) + 4 * h + (j % 4);
 }
 
-__device__ __forceinline__ __bf16 to_bf16(float f) { return (__bf16)f; }
+// Network operands in MFMA layout (v_mfma_f32_32x32x16_bf16: A lane l holds A[l%32][8(l/32)+j], B lane l
+// holds B[8(l/32)+j][l%32], C lane l holds C[8(r/4)+4(l/32)+r%4][l%32]; checked on the device).
+struct ClosurePacked {
+    bf16x8 a1[2 * 64];  // layer 1, M-tile mt: A[u][8h+j] = (h ? W1lo : W1hi)[u][j % 4], u = 32 mt + l%32
+    bf16x8 a2[8 * 64];  // layer 2, (mt, ks): A[m][8h+j] = W2[32 mt + l%32][chain_unit(ks, h, j)]
+    bf16x8 a3[4 * 64];  // layer 3, ks: A[m][8h+j] = m < 3 ? W3[m][chain_unit(ks, h, j)] : 0
+    float b1p[2][32];   // [h][16 mt + r] = b1[32 mt + 8 (r/4) + 4 h + r%4]
+    float b2p[2][32];
+    float b3[4];
+};
 
-// Network output (tau / mu) of the particle of every lane of the wave.  Must be called by all 64 lanes
-// (wave-uniform control flow); x holds the lane's own features.
-//
-// Layout (v_mfma_f32_32x32x16_bf16, checked on the device): A lane l holds A[l%32][8(l/32)+j],
-// B lane l holds B[8(l/32)+j][l%32], C lane l holds C[8(r/4)+4(l/32)+r%4][l%32].  Particles are the N
-// dimension: N-tile nt holds particles 32 nt + (l%32).  Layer 1 runs on the VALU directly in the B layout
-// (each lane evaluates the 32 hidden units it feeds to the MFMA, for both N-tiles); layer 2 is 2 M-tiles x
-// 4 K-steps of MFMA per N-tile with FP32 accumulation; layer 3 (FP32, VALU) reduces the 32 units a lane
-// holds, and the two half-waves' partial sums are combined with permlane32 swaps.
-__device__ __forceinline__ void closure_eval_wave(const ClosureLds* s, const float x[4], float y[3])
+__device__ __forceinline__ void closure_stage(const ClosureDev& c, ClosurePacked* s)
+{
+    const uint4* src = reinterpret_cast<const uint4*>(c.packed);
+    uint4* dst = reinterpret_cast<uint4*>(s);
+    constexpr int n16 = (int)(sizeof(ClosurePacked) / 16);
+    for (int i = threadIdx.x; i < n16; i += blockDim.x) { dst[i] = src[i]; }
+}
+
+// Keeps the LDS operand loads of one network stage next to their MFMAs: without it the compiler hoists
+// all 14 A operands and 64 biases out of the particle loop and the fused kernels lose occupancy.
+__device__ __forceinline__ void closure_stage_fence() { __asm__ __volatile__("" ::: "memory"); }
+
+// Accumulator of M-tile mt initialised with the layer's biases (C layout, this half-wave's rows).
+__device__ __forceinline__ floatx16 bias_acc(const float (*bias)[32], int half, int mt)
+{
+    const float4* b = reinterpret_cast<const float4*>(&bias[half][16 * mt]);
+    floatx16 acc;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const float4 v = b[q];
+        acc[4 * q] = v.x;
+        acc[4 * q + 1] = v.y;
+        acc[4 * q + 2] = v.z;
+        acc[4 * q + 3] = v.w;
+    }
+    return acc;
+}
+
+// Layer output -> next B operand: SiLU and BF16, for K-steps 2 mt and 2 mt + 1.
+__device__ __forceinline__ void chain_operands(const floatx16& acc, bf16x8& lo, bf16x8& hi)
+{
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        lo[j] = (__bf16)silu(acc[j]);
+        hi[j] = (__bf16)silu(acc[8 + j]);
+    }
+}
+
+// Network output (tau / mu) of the particle of every lane.  Must be called by all 64 lanes
+// (wave-uniform control flow); x holds the lane's own features.  Particles are the N dimension of the
+// MFMAs: N-tile nt holds particles 32 nt + l%32, evaluated one N-tile at a time.
+__device__ __forceinline__ void closure_eval_wave(const ClosurePacked* s, const float x[4], float y[3])
 {
     const int lane = threadIdx.x & 63;
     const int half = lane >> 5;
-    float part[2][3];
-#pragma unroll
+    float res[2][3];
+#pragma unroll 1
     for (int nt = 0; nt < 2; ++nt) {
-        // features of particle 32 nt + lane % 32 (s[0] = value of lane l%32, s[1] = value of lane l%32 + 32)
-        float f[4];
+        // layer 1 B operand: (x_hi, x_lo) of particle 32 nt + l%32 in both halves, so that
+        // A1 B1 = W1hi (x_hi + x_lo) + W1lo (x_hi + x_lo)
+        bf16x8 b1;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(x[k]), __float_as_uint(x[k]), false,
                                                              false);
-            f[k] = __uint_as_float(nt == 0 ? sw[0] : sw[1]);
+            const float f = __uint_as_float(nt == 0 ? sw[0] : sw[1]);
+            const __bf16 fh = (__bf16)f;
+            b1[k] = fh;
+            b1[4 + k] = (__bf16)(f - (float)fh);
         }
-        // layer 1 for the units this lane feeds: u = 16 ks + 8 half + j
-        bf16x8 bop[4];
-#pragma unroll
-        for (int ks = 0; ks < 4; ++ks) {
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                const int u = 16 * ks + 8 * half + j;
-                const float4 w = s->w1[u];
-                float z = s->b1[u];
-                z = fmaf(w.x, f[0], z);
-                z = fmaf(w.y, f[1], z);
-                z = fmaf(w.z, f[2], z);
-                z = fmaf(w.w, f[3], z);
-                bop[ks][j] = to_bf16(silu(z));
-            }
-        }
-        // layer 2: z2[unit][particle] = sum_k W2[unit][k] h1[k][particle]
-        float p0 = 0.f, p1 = 0.f, p2 = 0.f;
+        bf16x8 b2[4];
 #pragma unroll
         for (int mt = 0; mt < 2; ++mt) {
-            floatx16 acc = {};
+            closure_stage_fence();
+            floatx16 acc = bias_acc(s->b1p, half, mt);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(s->a1[mt * 64 + lane], b1, acc, 0, 0, 0);
+            chain_operands(acc, b2[2 * mt], b2[2 * mt + 1]);
+        }
+        bf16x8 b3[4];
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt) {
+            closure_stage_fence();
+            floatx16 acc = bias_acc(s->b2p, half, mt);
 #pragma unroll
             for (int ks = 0; ks < 4; ++ks) {
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(s->w2[(mt * 4 + ks) * 64 + lane], bop[ks], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(s->a2[(mt * 4 + ks) * 64 + lane], b2[ks], acc, 0, 0, 0);
             }
-            // layer 3 partial sums over the 16 units of this M-tile held by the lane
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int u = 32 * mt + 8 * (r >> 2) + 4 * half + (r & 3);
-                const float h = silu(acc[r] + s->b2[u]);
-                const float4 w3 = s->w3[u];
-                p0 = fmaf(w3.x, h, p0);
-                p1 = fmaf(w3.y, h, p1);
-                p2 = fmaf(w3.z, h, p2);
-            }
+            chain_operands(acc, b3[2 * mt], b3[2 * mt + 1]);
         }
-        part[nt][0] = p0;
-        part[nt][1] = p1;
-        part[nt][2] = p2;
+        closure_stage_fence();
+        floatx16 out = {};
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) {
+            out = __builtin_amdgcn_mfma_f32_32x32x16_bf16(s->a3[ks * 64 + lane], b3[ks], out, 0, 0, 0);
+        }
+        // rows 0..2 (the three outputs) sit in registers 0..2 of the lower half-wave
+        res[nt][0] = out[0];
+        res[nt][1] = out[1];
+        res[nt][2] = out[2];
     }
-    // the lane's own particle is in N-tile `half`; add the other half-wave's partial sum for it
+    closure_stage_fence();
+    // lower half: own particle l is in N-tile 0; upper half: own particle l is N-tile 1 of lane l - 32
 #pragma unroll
     for (int o = 0; o < 3; ++o) {
-        const auto s0 = __builtin_amdgcn_permlane32_swap(__float_as_uint(part[0][o]), __float_as_uint(part[0][o]),
-                                                         false, false);
-        const auto s1 = __builtin_amdgcn_permlane32_swap(__float_as_uint(part[1][o]), __float_as_uint(part[1][o]),
-                                                         false, false);
-        const float own = half == 0 ? part[0][o] : part[1][o];
-        const float other = half == 0 ? __uint_as_float(s0[1]) : __uint_as_float(s1[0]);
-        y[o] = own + other + s->b3[o];
+        const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(res[1][o]), __float_as_uint(res[1][o]), false,
+                                                         false);
+        y[o] = (half == 0 ? res[0][o] : __uint_as_float(sw[0])) + s->b3[o];
     }
 }
 
 }  // namespace mpm
+

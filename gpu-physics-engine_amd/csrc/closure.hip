@@ -226,23 +226,43 @@ __global__ void k_sum(const float* a, int n, float* out)
     }
 }
 
-// W2 packed as the MFMA A operand: entry (mt*4 + ks)*64 + lane holds W2[32 mt + lane%32][16 ks + 8 (lane/32) + j].
-__global__ void k_pack_w2(const float* w, bf16x8* out)
+// The network in MFMA operand layout (ClosurePacked): one thread per 16-byte A-operand entry or bias.
+__global__ void k_pack_closure(const float* w, ClosurePacked* out)
 {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= 8 * 64) { return; }
-    const int lane = e % 64, blk = e / 64, mt = blk / 4, ks = blk % 4;
+    const int lane = e % 64, m = lane % 32, h = lane / 32;
     bf16x8 v;
-    for (int j = 0; j < 8; ++j) {
-        v[j] = (__bf16)w[kOffW2 + kHid * (32 * mt + lane % 32) + 16 * ks + 8 * (lane / 32) + j];
+    if (e < 2 * 64) {
+        const int u = 32 * (e / 64) + m;
+        for (int j = 0; j < 8; ++j) {
+            const float f = w[kOffW1 + 4 * u + j % 4];
+            const __bf16 hi = (__bf16)f;
+            v[j] = h ? (__bf16)(f - (float)hi) : hi;
+        }
+        out->a1[e] = v;
+    } else if (e < 10 * 64) {
+        const int blk = e / 64 - 2, mt = blk / 4, ks = blk % 4;
+        for (int j = 0; j < 8; ++j) { v[j] = (__bf16)w[kOffW2 + kHid * (32 * mt + m) + chain_unit(ks, h, j)]; }
+        out->a2[e - 2 * 64] = v;
+    } else if (e < 14 * 64) {
+        const int ks = e / 64 - 10;
+        for (int j = 0; j < 8; ++j) { v[j] = (__bf16)(m < 3 ? w[kOffW3 + kHid * m + chain_unit(ks, h, j)] : 0.f); }
+        out->a3[e - 10 * 64] = v;
+    } else if (e < 14 * 64 + 128) {
+        // bias of C register r of M-tile mt in half-wave hh
+        const int b = e - 14 * 64, layer = b / 64, hh = (b / 32) % 2, mt = (b / 16) % 2, r = b % 16;
+        const float bias = w[(layer ? kOffB2 : kOffB1) + 32 * mt + 8 * (r / 4) + 4 * hh + r % 4];
+        (layer ? out->b2p : out->b1p)[hh][16 * mt + r] = bias;
+    } else if (e < 14 * 64 + 132) {
+        const int o = e - 14 * 64 - 128;
+        out->b3[o] = o < 3 ? w[kOffB3 + o] : 0.f;
     }
-    out[e] = v;
 }
 
 // Network output of n feature vectors through the wave-level BF16 MFMA path used by the fused kernels.
 __global__ __launch_bounds__(kThreads) void k_closure_eval(ClosureDev c, int n, const float* X, float* Y)
 {
-    __shared__ ClosureLds lds;
+    __shared__ ClosurePacked lds;
     closure_stage(c, &lds);
     __syncthreads();
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -255,9 +275,12 @@ __global__ __launch_bounds__(kThreads) void k_closure_eval(ClosureDev c, int n, 
     }
 }
 
-hipError_t launch_pack_w2(hipStream_t st, const float* w, void* out)
+size_t closure_packed_bytes() { return sizeof(ClosurePacked); }
+
+hipError_t launch_pack_closure(hipStream_t st, const float* w, void* out)
 {
-    hipLaunchKernelGGL(k_pack_w2, dim3(2), dim3(kThreads), 0, st, w, reinterpret_cast<bf16x8*>(out));
+    hipLaunchKernelGGL(k_pack_closure, dim3((14 * 64 + 132 + kThreads - 1) / kThreads), dim3(kThreads), 0, st, w,
+                       reinterpret_cast<ClosurePacked*>(out));
     return hipGetLastError();
 }
 
@@ -457,16 +480,16 @@ int32_t mpm_closure_train(int32_t n, const float* features, const float* targets
 int32_t mpm_closure_eval(int32_t n, const float* features, const float* weights, float* out)
 {
     if (n < 1 || !features || !weights || !out) { return MPM_ERR_INVALID_ARG; }
-    DevBuf b, w2;
+    DevBuf b, packed;
     CL_TRY(hipMalloc(&b.p, ((size_t)n * 7 + kNumWeights) * sizeof(float)));
-    CL_TRY(hipMalloc(&w2.p, 8 * 64 * sizeof(bf16x8)));
+    CL_TRY(hipMalloc(&packed.p, sizeof(ClosurePacked)));
     float* dX = b.as<float>();
     float* dY = dX + (size_t)n * 4;
     float* dW = dY + (size_t)n * 3;
     CL_TRY(hipMemcpy(dX, features, (size_t)n * 4 * sizeof(float), hipMemcpyHostToDevice));
     CL_TRY(hipMemcpy(dW, weights, kNumWeights * sizeof(float), hipMemcpyHostToDevice));
-    CL_TRY(launch_pack_w2(0, dW, w2.p));
-    ClosureDev c{dW, w2.as<bf16x8>()};
+    CL_TRY(launch_pack_closure(0, dW, packed.p));
+    ClosureDev c{dW, packed.p};
     hipLaunchKernelGGL(k_closure_eval, dim3(nblk(n)), dim3(kThreads), 0, 0, c, n, dX, dY);
     CL_TRY(hipGetLastError());
     CL_TRY(hipMemcpy(out, dY, (size_t)n * 3 * sizeof(float), hipMemcpyDeviceToHost));

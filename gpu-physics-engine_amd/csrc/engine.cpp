@@ -49,7 +49,7 @@ struct MpmEngine {
     int32_t last_count = 0;
     int shape = 2;
     float* closure_w = nullptr;   // device float[4675]
-    void* closure_w2 = nullptr;   // device W2 in the MFMA operand layout (8 KiB)
+    void* closure_w2 = nullptr;   // device network in the MFMA operand layout (about 15 KiB)
     bool closure_loaded = false;
     bool has_ticked = false;
     int32_t last_error = 0;
@@ -221,7 +221,7 @@ int32_t allocate(MpmEngine* h)
         ss.num_items = c.take<int32_t>(1);
         h->count = c.take<int32_t>(2);
         h->closure_w = c.take<float>(4675);
-        h->closure_w2 = c.take<uint8_t>(8 * 64 * 16);
+        h->closure_w2 = c.take<uint8_t>(closure_packed_bytes());
         for (int k = 0; k < 3; ++k) {
             h->out_pos[k] = c.take<float>(2 * cap);
             h->out_vel[k] = c.take<float>(2 * cap);
@@ -235,7 +235,7 @@ int32_t allocate(MpmEngine* h)
     // Zero the whole slab once (grid accumulators must start at zero; counts at zero).
     HIP_TRY(h, hipMemsetAsync(h->slab, 0, h->slab_bytes, h->stream));
     h->dp.cl.w = h->closure_w;
-    h->dp.cl.w2_mfma = reinterpret_cast<const bf16x8*>(h->closure_w2);
+    h->dp.cl.packed = h->closure_w2;
     HIP_TRY(h, hipHostMalloc((void**)&h->staged, 3 * sizeof(OutView), hipHostMallocDefault));
     for (int k = 0; k < 3; ++k) {
         h->staged[k].pos = h->out_pos[k];
@@ -775,7 +775,7 @@ int32_t mpm_set_closure(MpmEngine* h, const float* weights)
     }
     HIP_TRY(h, hipStreamSynchronize(h->stream));
     HIP_TRY(h, hipMemcpy(h->closure_w, weights, 4675 * sizeof(float), hipMemcpyHostToDevice));
-    HIP_TRY(h, launch_pack_w2(h->stream, h->closure_w, h->closure_w2));
+    HIP_TRY(h, launch_pack_closure(h->stream, h->closure_w, h->closure_w2));
     HIP_TRY(h, hipStreamSynchronize(h->stream));
     h->closure_loaded = true;
     return MPM_OK;

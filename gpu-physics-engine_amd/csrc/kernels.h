@@ -68,7 +68,8 @@ hipError_t launch_g2p2g(hipStream_t st, int shape, const DevParams& p, const Sto
 
 hipError_t launch_output(hipStream_t st, const Store& s, int32_t* count, int cap);
 // Pack W2 of the closure (float[4675] parameters) into the MFMA A-operand layout (8 x 64 bf16x8).
-hipError_t launch_pack_w2(hipStream_t st, const float* w, void* out);
+size_t closure_packed_bytes();
+hipError_t launch_pack_closure(hipStream_t st, const float* w, void* out);
 hipError_t set_output_view_async(hipStream_t st, const OutView* host_pinned);
 
 hipError_t launch_emitter(hipStream_t st, const DevParams& p, const Store& s, int32_t* count, int cap, float ox,

@@ -909,7 +909,7 @@ __global__ __launch_bounds__(kThreads) void k_g2p2g(DevParams p, Store s, Grid g
 // Affine momentum with the learned closure: jelly and snow particles inside the trained range take
 // tau = mu * MLP(features); fluid and out-of-range particles the analytic stress.  Wave-uniform: the
 // network runs once per wave when any lane needs it (all 64 lanes must be active).
-__device__ __forceinline__ Mat2 affine_q_learned(const DevParams& p, const ClosureLds* cl, int mat, const Mat2& C,
+__device__ __forceinline__ Mat2 affine_q_learned(const DevParams& p, const ClosurePacked* cl, int mat, const Mat2& C,
                                                  const Mat2& F, float J)
 {
     float x[4] = {0.f, 0.f, 0.f, 0.f};
@@ -933,7 +933,7 @@ __global__ __launch_bounds__(kThreads) void k_p2g_tiled_learned(DevParams p, Sto
 {
     using T = TileShape<BX, BY>;
     __shared__ float4 acc[T::N * (kThreads / 64)];
-    __shared__ ClosureLds cl;
+    __shared__ ClosurePacked cl;
     bool ok;
     const int it = xcd_item(num_items[0], ok);
     if (!ok) { return; }
@@ -968,7 +968,7 @@ __global__ __launch_bounds__(kThreads) void k_g2p2g_learned(DevParams p, Store s
     using T = TileShape<BX, BY>;
     __shared__ float2 sv[T::N];
     __shared__ float4 acc[T::N * (kThreads / 64)];
-    __shared__ ClosureLds cl;
+    __shared__ ClosurePacked cl;
     bool ok;
     const int it = xcd_item(num_items[0], ok);
     if (!ok) { return; }

@@ -25,8 +25,8 @@ typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 
 // Device copy of the learned closure's parameters (see closure.h).
 struct ClosureDev {
-    const float* w;         // float[4675]
-    const bf16x8* w2_mfma;  // W2 packed as the MFMA A operand
+    const float* w;       // float[4675]
+    const void* packed;   // the network in MFMA operand layout (ClosurePacked)
 };
 
 // Kernel-argument copy of MpmParams plus derived quantities (passed by value, lives in SGPRs/kernarg).

@@ -24,12 +24,16 @@ def silu(z):
 
 
 def forward_bf16(X, w):
-    """Restatement of the device inference: FP32 layer 1, BF16 operands of layer 2 with FP32 accumulation,
-    FP32 layer 3."""
+    """Restatement of the device inference: layer 1 with operands split into BF16 high and low parts (FP32
+    accurate), BF16 operands of layers 2 and 3 with FP32 accumulation and FP32 biases."""
     P = closure.unflatten(w)
-    h1 = silu(X.astype(np.float64) @ P["W1"].T + P["b1"])
+
+    def split(a):
+        hi = bf16(a)
+        return hi.astype(np.float64) + bf16(a - hi).astype(np.float64)
+    h1 = silu(split(X) @ split(P["W1"]).T + P["b1"])
     z2 = bf16(h1).astype(np.float64) @ bf16(P["W2"]).astype(np.float64).T + P["b2"]
-    return silu(z2) @ P["W3"].T + P["b3"]
+    return bf16(silu(z2)).astype(np.float64) @ bf16(P["W3"]).astype(np.float64).T + P["b3"]
 
 
 def test_gradient_matches_fp64_oracle(oracle_mod):
