@@ -26,11 +26,11 @@ constexpr float kSigLo = 0.6f, kSigHi = 1.6f, kJhLo = 0.7f, kJhHi = 1.3f;
 
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float floatx16;
 
-__device__ __forceinline__ float silu(float z)
-{
-    // z * sigmoid(z) with the hardware exp2 and reciprocal
-    return z * __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(-1.44269504f * z));
-}
+// The packed network computes every hidden pre-activation in the base-2 scale s = -log2(e) z; then
+// silu(z) = z / (1 + e^-z) = -ln 2 * s / (1 + 2^s).  The factor -ln 2 is folded into the next layer:
+// W2 (-ln 2)(-log2 e) = W2 and W3 (-ln 2) (see k_pack_closure), so the activation is two VALU operations
+// and the hardware exp2 and reciprocal.
+__device__ __forceinline__ float silu_base2(float s) { return s * __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(s)); }
 
 // atan2(y, x): atan on [0, 1] from the degree-9 odd polynomial of Abramowitz & Stegun 4.4.47 (|error| <=
 // 1e-5 rad) plus octant reduction.  Training and inference use this same function, so the network's angle
@@ -86,11 +86,12 @@ __host__ __device__ constexpr int chain_unit(int ks, int h, int j)
 // Network operands in MFMA layout (v_mfma_f32_32x32x16_bf16: A lane l holds A[l%32][8(l/32)+j], B lane l
 // holds B[8(l/32)+j][l%32], C lane l holds C[8(r/4)+4(l/32)+r%4][l%32]; checked on the device).
 struct ClosurePacked {
+    // with c = -log2(e), W1s = c W1 split as W1s = W1hi + W1lo (BF16 parts):
     bf16x8 a1[2 * 64];  // layer 1, M-tile mt: A[u][8h+j] = (h ? W1lo : W1hi)[u][j % 4], u = 32 mt + l%32
     bf16x8 a2[8 * 64];  // layer 2, (mt, ks): A[m][8h+j] = W2[32 mt + l%32][chain_unit(ks, h, j)]
-    bf16x8 a3[4 * 64];  // layer 3, ks: A[m][8h+j] = m < 3 ? W3[m][chain_unit(ks, h, j)] : 0
-    float b1p[2][32];   // [h][16 mt + r] = b1[32 mt + 8 (r/4) + 4 h + r%4]
-    float b2p[2][32];
+    bf16x8 a3[4 * 64];  // layer 3, ks: A[m][8h+j] = m < 3 ? -ln 2 W3[m][chain_unit(ks, h, j)] : 0
+    float b1p[2][32];   // [h][16 mt + r] = c b1[32 mt + 8 (r/4) + 4 h + r%4]
+    float b2p[2][32];   // likewise c b2
     float b3[4];
 };
 
@@ -122,13 +123,19 @@ __device__ __forceinline__ floatx16 bias_acc(const float (*bias)[32], int half, 
     return acc;
 }
 
-// Layer output -> next B operand: SiLU and BF16, for K-steps 2 mt and 2 mt + 1.
+// Layer output -> next B operand: SiLU and BF16, for K-steps 2 mt and 2 mt + 1.  The 16 activations are
+// computed stage by stage (all exp2, then all reciprocals) so their transcendental latencies overlap.
 __device__ __forceinline__ void chain_operands(const floatx16& acc, bf16x8& lo, bf16x8& hi)
 {
+    float e[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) { e[j] = __builtin_amdgcn_exp2f(acc[j]); }
+#pragma unroll
+    for (int j = 0; j < 16; ++j) { e[j] = __builtin_amdgcn_rcpf(1.f + e[j]); }
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-        lo[j] = (__bf16)silu(acc[j]);
-        hi[j] = (__bf16)silu(acc[8 + j]);
+        lo[j] = (__bf16)(acc[j] * e[j]);
+        hi[j] = (__bf16)(acc[8 + j] * e[8 + j]);
     }
 }
 

@@ -232,10 +232,12 @@ __global__ void k_pack_closure(const float* w, ClosurePacked* out)
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = e % 64, m = lane % 32, h = lane / 32;
     bf16x8 v;
+    constexpr float kC = -1.44269504088896341f;  // -log2(e): hidden pre-activations in base-2 scale
+    constexpr float kL = -0.69314718055994531f;  // -ln(2): folded into the output layer
     if (e < 2 * 64) {
         const int u = 32 * (e / 64) + m;
         for (int j = 0; j < 8; ++j) {
-            const float f = w[kOffW1 + 4 * u + j % 4];
+            const float f = kC * w[kOffW1 + 4 * u + j % 4];
             const __bf16 hi = (__bf16)f;
             v[j] = h ? (__bf16)(f - (float)hi) : hi;
         }
@@ -246,12 +248,12 @@ __global__ void k_pack_closure(const float* w, ClosurePacked* out)
         out->a2[e - 2 * 64] = v;
     } else if (e < 14 * 64) {
         const int ks = e / 64 - 10;
-        for (int j = 0; j < 8; ++j) { v[j] = (__bf16)(m < 3 ? w[kOffW3 + kHid * m + chain_unit(ks, h, j)] : 0.f); }
+        for (int j = 0; j < 8; ++j) { v[j] = (__bf16)(m < 3 ? kL * w[kOffW3 + kHid * m + chain_unit(ks, h, j)] : 0.f); }
         out->a3[e - 10 * 64] = v;
     } else if (e < 14 * 64 + 128) {
         // bias of C register r of M-tile mt in half-wave hh
         const int b = e - 14 * 64, layer = b / 64, hh = (b / 32) % 2, mt = (b / 16) % 2, r = b % 16;
-        const float bias = w[(layer ? kOffB2 : kOffB1) + 32 * mt + 8 * (r / 4) + 4 * hh + r % 4];
+        const float bias = kC * w[(layer ? kOffB2 : kOffB1) + 32 * mt + 8 * (r / 4) + 4 * hh + r % 4];
         (layer ? out->b2p : out->b1p)[hh][16 * mt + r] = bias;
     } else if (e < 14 * 64 + 132) {
         const int o = e - 14 * 64 - 128;

@@ -24,16 +24,23 @@ def silu(z):
 
 
 def forward_bf16(X, w):
-    """Restatement of the device inference: layer 1 with operands split into BF16 high and low parts (FP32
-    accurate), BF16 operands of layers 2 and 3 with FP32 accumulation and FP32 biases."""
+    """Restatement of the device inference: hidden pre-activations in base-2 scale s = -log2(e) z, so that
+    silu(z) = -ln 2 * s / (1 + 2^s) with -ln 2 folded into the next layer's weights; layer 1 with operands
+    split into BF16 high and low parts (FP32 accurate), BF16 operands of layers 2 and 3 with FP32
+    accumulation and FP32 biases."""
     P = closure.unflatten(w)
+    c = np.float32(-np.log2(np.e))
+    ln2 = np.float32(-np.log(2.0))
 
     def split(a):
         hi = bf16(a)
         return hi.astype(np.float64) + bf16(a - hi).astype(np.float64)
-    h1 = silu(split(X) @ split(P["W1"]).T + P["b1"])
-    z2 = bf16(h1).astype(np.float64) @ bf16(P["W2"]).astype(np.float64).T + P["b2"]
-    return bf16(silu(z2)).astype(np.float64) @ bf16(P["W3"]).astype(np.float64).T + P["b3"]
+
+    def act(s):
+        return s / (1.0 + np.exp2(s))
+    s1 = split(X) @ split(c * P["W1"]).T + (c * P["b1"]).astype(np.float64)
+    s2 = bf16(act(s1)).astype(np.float64) @ bf16(P["W2"]).astype(np.float64).T + (c * P["b2"]).astype(np.float64)
+    return bf16(act(s2)).astype(np.float64) @ bf16(ln2 * P["W3"]).astype(np.float64).T + P["b3"]
 
 
 def test_gradient_matches_fp64_oracle(oracle_mod):
