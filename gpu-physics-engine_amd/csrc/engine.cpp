@@ -11,6 +11,7 @@
 
 #include <atomic>
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -46,7 +47,8 @@ struct MpmEngine {
     std::atomic<int32_t> read_slot{-1};
     int32_t write_slot = 0;
     int32_t pending_slot = -1;
-    int32_t last_count = 0;
+    int32_t last_count = 0;             // particle count as of the last enqueued work (host-tracked)
+    std::atomic<int32_t> slot_count[3]{};  // particles written to each output slot
     int shape = 2;
     float* closure_w = nullptr;   // device float[4675]
     void* closure_w2 = nullptr;   // device network in the MFMA operand layout (about 15 KiB)
@@ -369,6 +371,107 @@ void release(MpmEngine* h)
 
 float f32_ref_dt() { return 0.032f / 20.f; }
 
+// Upload a population; scene[i] is the scene of particle i (any order).  Particle i keeps id i.
+int32_t set_particles_impl(MpmEngine* h, int32_t n, const float* x, const float* v, const float* C, const float* F,
+                           const float* J, const uint8_t* material, const uint32_t* scene)
+{
+    const MpmParams& p = h->params;
+    std::vector<uint32_t> scene_key((size_t)n, 0u);
+    for (int32_t i = 0; i < n; ++i) {
+        if (scene[i] >= (uint32_t)p.num_scenes) { return fail(h, MPM_ERR_INVALID_ARG, 0, "scene index out of range"); }
+        scene_key[(size_t)i] = scene[i] * (uint32_t)h->dp.bps;
+    }
+    for (int32_t i = 0; i < n; ++i) {
+        const float px = x[2 * i], py = x[2 * i + 1];
+        if (!(px >= 0.f && px <= p.sim_w && py >= 0.f && py <= p.sim_h)) {
+            return fail(h, MPM_ERR_INVALID_ARG, 0, "particle position outside [0,sim_w]x[0,sim_h] or not finite");
+        }
+        if (material && material[i] > MPM_MAT_SNOW) { return fail(h, MPM_ERR_INVALID_ARG, 0, "unknown material"); }
+    }
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    const size_t N = (size_t)n;
+    std::vector<float> planes(13 * N);
+    for (size_t i = 0; i < N; ++i) {
+        planes[0 * N + i] = x[2 * i];
+        planes[1 * N + i] = x[2 * i + 1];
+        planes[2 * N + i] = v ? v[2 * i] : 0.f;
+        planes[3 * N + i] = v ? v[2 * i + 1] : 0.f;
+        for (int k = 0; k < 4; ++k) {
+            planes[(4 + k) * N + i] = C ? C[4 * i + k] : 0.f;
+            planes[(8 + k) * N + i] = F ? F[4 * i + k] : ((k == 0 || k == 3) ? 1.f : 0.f);
+        }
+        planes[12 * N + i] = J ? J[i] : 1.f;
+    }
+    std::vector<uint8_t> mat(N);
+    std::vector<int32_t> ids(N);
+    for (size_t i = 0; i < N; ++i) {
+        mat[i] = material ? material[i] : (uint8_t)MPM_MAT_FLUID;
+        ids[i] = (int32_t)i;
+    }
+    Store& s = h->store[h->cur];
+    float* dst[13] = {s.x, s.y, s.vx, s.vy, s.c00, s.c01, s.c10, s.c11, s.f00, s.f01, s.f10, s.f11, s.J};
+    if (n > 0) {
+        for (int k = 0; k < 13; ++k) {
+            HIP_TRY(h, hipMemcpy(dst[k], planes.data() + (size_t)k * N, N * sizeof(float), hipMemcpyHostToDevice));
+        }
+        HIP_TRY(h, hipMemcpy(s.mat, mat.data(), N, hipMemcpyHostToDevice));
+        HIP_TRY(h, hipMemcpy(s.id, ids.data(), N * sizeof(int32_t), hipMemcpyHostToDevice));
+        HIP_TRY(h, hipMemcpy(s.key, scene_key.data(), N * sizeof(uint32_t), hipMemcpyHostToDevice));
+    }
+    const int32_t cnt[2] = {n, n};
+    HIP_TRY(h, hipMemcpy(h->count, cnt, sizeof(cnt), hipMemcpyHostToDevice));
+    h->last_count = n;
+    return MPM_OK;
+}
+
+
+
+// ---- state file (mpm_save_state / mpm_load_state) ----
+// Little-endian: StateHeader, then the payload x f32[2n] | v f32[2n] | C f32[4n] | F f32[4n] | J f32[n] |
+// scene i32[n] | material u8[n], checksummed with 64-bit FNV-1a.
+constexpr char kStateMagic[8] = {'M', 'P', 'M', 'S', 'T', 'A', 'T', 'E'};
+constexpr uint32_t kStateVersion = 1;
+
+struct StateHeader {
+    char magic[8];
+    uint32_t version;
+    uint32_t header_bytes;
+    int32_t n;
+    int32_t num_scenes;
+    uint64_t checksum;
+    MpmParams params;
+    uint32_t reserved;  // zero; pads the header to a multiple of 8 bytes
+};
+static_assert(sizeof(MpmParams) == 164 && sizeof(StateHeader) == 200, "state header layout");
+static_assert(offsetof(StateHeader, params) == 32, "state header layout");
+
+uint64_t fnv1a64(const uint8_t* b, size_t n)
+{
+    uint64_t hsh = 0xcbf29ce484222325ull;  // FNV-1a 64-bit offset basis
+    for (size_t i = 0; i < n; ++i) {
+        hsh ^= b[i];
+        hsh *= 0x100000001b3ull;  // FNV 64-bit prime
+    }
+    return hsh;
+}
+
+struct StatePayload {
+    std::vector<uint8_t> bytes;
+    float *x, *v, *C, *F, *J;
+    uint8_t *scene, *mat;
+    explicit StatePayload(size_t n) : bytes(n * 57)
+    {
+        float* f = reinterpret_cast<float*>(bytes.data());
+        x = f;
+        v = x + 2 * n;
+        C = v + 2 * n;
+        F = C + 4 * n;
+        J = F + 4 * n;
+        scene = reinterpret_cast<uint8_t*>(J + n);
+        mat = scene + 4 * n;
+    }
+};
+
 }  // namespace
 
 extern "C" {
@@ -554,60 +657,21 @@ int32_t mpm_set_particles(MpmEngine* h, int32_t n, const float* x, const float* 
     if (n < 0 || (n > 0 && !x)) { return fail(h, MPM_ERR_INVALID_ARG, 0, "n < 0 or x is NULL"); }
     if (n > p.capacity) { return fail(h, MPM_ERR_CAPACITY, 0, "n exceeds capacity"); }
     if (p.num_scenes > 1 && !scene_counts) { return fail(h, MPM_ERR_INVALID_ARG, 0, "scene_counts required"); }
-    std::vector<uint32_t> scene_key((size_t)n, 0u);
+    std::vector<uint32_t> scene((size_t)n, 0u);
     if (scene_counts) {
         int64_t total = 0;
         for (int s = 0; s < p.num_scenes; ++s) {
             if (scene_counts[s] < 0) { return fail(h, MPM_ERR_INVALID_ARG, 0, "negative scene count"); }
             for (int32_t k = 0; k < scene_counts[s]; ++k) {
-                if (total + k < n) { scene_key[(size_t)(total + k)] = (uint32_t)s * (uint32_t)h->dp.bps; }
+                if (total + k < n) { scene[(size_t)(total + k)] = (uint32_t)s; }
             }
             total += scene_counts[s];
         }
         if (total != n) { return fail(h, MPM_ERR_INVALID_ARG, 0, "scene_counts do not sum to n"); }
     }
-    for (int32_t i = 0; i < n; ++i) {
-        const float px = x[2 * i], py = x[2 * i + 1];
-        if (!(px >= 0.f && px <= p.sim_w && py >= 0.f && py <= p.sim_h)) {
-            return fail(h, MPM_ERR_INVALID_ARG, 0, "particle position outside [0,sim_w]x[0,sim_h] or not finite");
-        }
-        if (material && material[i] > MPM_MAT_SNOW) { return fail(h, MPM_ERR_INVALID_ARG, 0, "unknown material"); }
-    }
-    HIP_TRY(h, hipStreamSynchronize(h->stream));
-    const size_t N = (size_t)n;
-    std::vector<float> planes(13 * N);
-    for (size_t i = 0; i < N; ++i) {
-        planes[0 * N + i] = x[2 * i];
-        planes[1 * N + i] = x[2 * i + 1];
-        planes[2 * N + i] = v ? v[2 * i] : 0.f;
-        planes[3 * N + i] = v ? v[2 * i + 1] : 0.f;
-        for (int k = 0; k < 4; ++k) {
-            planes[(4 + k) * N + i] = C ? C[4 * i + k] : 0.f;
-            planes[(8 + k) * N + i] = F ? F[4 * i + k] : ((k == 0 || k == 3) ? 1.f : 0.f);
-        }
-        planes[12 * N + i] = J ? J[i] : 1.f;
-    }
-    std::vector<uint8_t> mat(N);
-    std::vector<int32_t> ids(N);
-    for (size_t i = 0; i < N; ++i) {
-        mat[i] = material ? material[i] : (uint8_t)MPM_MAT_FLUID;
-        ids[i] = (int32_t)i;
-    }
-    Store& s = h->store[h->cur];
-    float* dst[13] = {s.x, s.y, s.vx, s.vy, s.c00, s.c01, s.c10, s.c11, s.f00, s.f01, s.f10, s.f11, s.J};
-    if (n > 0) {
-        for (int k = 0; k < 13; ++k) {
-            HIP_TRY(h, hipMemcpy(dst[k], planes.data() + (size_t)k * N, N * sizeof(float), hipMemcpyHostToDevice));
-        }
-        HIP_TRY(h, hipMemcpy(s.mat, mat.data(), N, hipMemcpyHostToDevice));
-        HIP_TRY(h, hipMemcpy(s.id, ids.data(), N * sizeof(int32_t), hipMemcpyHostToDevice));
-        HIP_TRY(h, hipMemcpy(s.key, scene_key.data(), N * sizeof(uint32_t), hipMemcpyHostToDevice));
-    }
-    const int32_t cnt[2] = {n, n};
-    HIP_TRY(h, hipMemcpy(h->count, cnt, sizeof(cnt), hipMemcpyHostToDevice));
-    h->last_count = n;
-    return MPM_OK;
+    return set_particles_impl(h, n, x, v, C, F, J, material, scene.data());
 }
+
 
 int32_t mpm_step(MpmEngine* h, int32_t n_ticks)
 {
@@ -624,6 +688,7 @@ int32_t mpm_step(MpmEngine* h, int32_t n_ticks)
         HIP_TRY(h, hipGraphLaunch(h->exec[h->cur], h->stream));
         if (last) { HIP_TRY(h, hipEventRecord(h->t1, h->stream)); }
         h->cur ^= h->flips;
+        h->slot_count[slot].store(h->last_count, std::memory_order_relaxed);
         h->pending_slot = slot;
         h->write_slot = (slot + 1) % 3;
         h->has_ticked = true;
@@ -674,7 +739,7 @@ int32_t mpm_read_view(MpmEngine* h, MpmView* out)
     out->pos = h->out_pos[s];
     out->vel = h->out_vel[s];
     out->num_atoms = h->out_num[s];
-    out->count = h->last_count;
+    out->count = h->slot_count[s].load(std::memory_order_relaxed);
     return MPM_OK;
 }
 
@@ -729,6 +794,85 @@ int32_t mpm_readback(MpmEngine* h, float* x, float* v, float* C, float* F, float
     return MPM_OK;
 }
 
+int32_t mpm_save_state(MpmEngine* h, const char* path)
+{
+    if (!h || !path) { return MPM_ERR_INVALID_ARG; }
+    int32_t cnt = 0;
+    int32_t rc = mpm_count(h, &cnt);
+    if (rc != MPM_OK) { return rc; }
+    const size_t N = (size_t)cnt;
+    StatePayload sp(N);
+    if (N > 0) {
+        rc = mpm_readback(h, sp.x, sp.v, sp.C, sp.F, sp.J, sp.mat, nullptr);
+        if (rc != MPM_OK) { return rc; }
+        const Store& st = h->store[h->cur];
+        std::vector<int32_t> ids(N);
+        std::vector<uint32_t> keys(N);
+        HIP_TRY(h, hipMemcpy(ids.data(), st.id, N * sizeof(int32_t), hipMemcpyDeviceToHost));
+        HIP_TRY(h, hipMemcpy(keys.data(), st.key, N * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        for (size_t j = 0; j < N; ++j) {
+            const int32_t sc = h->params.num_scenes > 1 ? (int32_t)(keys[j] / (uint32_t)h->dp.bps) : 0;
+            std::memcpy(sp.scene + 4 * (size_t)ids[j], &sc, 4);
+        }
+    }
+    StateHeader hd{};
+    std::memcpy(hd.magic, kStateMagic, 8);
+    hd.version = kStateVersion;
+    hd.header_bytes = (uint32_t)sizeof(StateHeader);
+    hd.n = cnt;
+    hd.num_scenes = h->params.num_scenes;
+    hd.checksum = fnv1a64(sp.bytes.data(), sp.bytes.size());
+    hd.params = h->params;
+    FILE* f = std::fopen(path, "wb");
+    if (!f) { return fail(h, MPM_ERR_INVALID_ARG, 0, std::string("cannot open ") + path + " for writing"); }
+    bool ok = std::fwrite(&hd, sizeof(hd), 1, f) == 1;
+    ok = ok && (sp.bytes.empty() || std::fwrite(sp.bytes.data(), 1, sp.bytes.size(), f) == sp.bytes.size());
+    ok = (std::fclose(f) == 0) && ok;
+    return ok ? MPM_OK : fail(h, MPM_ERR_INVALID_ARG, 0, std::string("write failed: ") + path);
+}
+
+int32_t mpm_load_state(MpmEngine* h, const char* path)
+{
+    if (!h || !path) { return MPM_ERR_INVALID_ARG; }
+    FILE* f = std::fopen(path, "rb");
+    if (!f) { return fail(h, MPM_ERR_INVALID_ARG, 0, std::string("cannot open ") + path); }
+    StateHeader hd{};
+    const bool got = std::fread(&hd, sizeof(hd), 1, f) == 1;
+    auto bad = [&](const std::string& why) {
+        std::fclose(f);
+        return fail(h, MPM_ERR_INVALID_ARG, 0, std::string("state file ") + path + ": " + why);
+    };
+    if (!got || std::memcmp(hd.magic, kStateMagic, 8) != 0) { return bad("not an MPM state file"); }
+    if (hd.version != kStateVersion || hd.header_bytes != sizeof(StateHeader)) { return bad("unsupported version"); }
+    const MpmParams& p = h->params;
+    if (hd.num_scenes != p.num_scenes || hd.params.grid_w != p.grid_w || hd.params.grid_h != p.grid_h) {
+        return bad("scene count or grid size differs from the handle's");
+    }
+    if (hd.n < 0) { return bad("negative particle count"); }
+    if (hd.n > p.capacity) {
+        std::fclose(f);
+        return fail(h, MPM_ERR_CAPACITY, 0, "state file holds more particles than the handle's capacity");
+    }
+    StatePayload sp((size_t)hd.n);
+    if (!sp.bytes.empty() && std::fread(sp.bytes.data(), 1, sp.bytes.size(), f) != sp.bytes.size()) {
+        return bad("truncated");
+    }
+    if (std::fgetc(f) != EOF) { return bad("trailing bytes"); }
+    std::fclose(f);
+    if (fnv1a64(sp.bytes.data(), sp.bytes.size()) != hd.checksum) {
+        return fail(h, MPM_ERR_INVALID_ARG, 0, std::string("state file ") + path + ": checksum mismatch");
+    }
+    std::vector<uint32_t> scene((size_t)hd.n);
+    for (size_t i = 0; i < (size_t)hd.n; ++i) {
+        int32_t sc;
+        std::memcpy(&sc, sp.scene + 4 * i, 4);
+        if (sc < 0 || sc >= p.num_scenes) { return fail(h, MPM_ERR_INVALID_ARG, 0, "state file: scene out of range"); }
+        scene[i] = (uint32_t)sc;
+    }
+    // float arrays live at 4-byte aligned offsets of the payload
+    return set_particles_impl(h, hd.n, sp.x, sp.v, sp.C, sp.F, sp.J, sp.mat, scene.data());
+}
+
 int32_t mpm_read_grid(MpmEngine* h, float* mass, float* vel_x, float* vel_y)
 {
     if (!h) { return MPM_ERR_INVALID_ARG; }
@@ -763,6 +907,8 @@ int32_t mpm_hose_pipe(MpmEngine* h, float origin_x, float origin_y, float veloci
                                         origin_y, velocity_x, velocity_y, width, spacing, jitter, count, tick,
                                         1.f / tick_len);
     if (e != hipSuccess) { return fail(h, MPM_ERR_DEVICE, (int32_t)e, "emitter launch"); }
+    // the device keeps the first capacity - count particles (k_emitter), so the host count stays exact
+    h->last_count = (int32_t)std::min<int64_t>(h->params.capacity, (int64_t)h->last_count + count);
     if (emitted) { *emitted = count; }
     return MPM_OK;
 }

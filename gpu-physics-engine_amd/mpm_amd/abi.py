@@ -109,6 +109,17 @@ class MpmTuneResult(ctypes.Structure):
     ]
 
 
+class MpmRenderDesc(ctypes.Structure):
+    _fields_ = [
+        ("width", ctypes.c_int32),
+        ("height", ctypes.c_int32),
+        ("row_pitch", ctypes.c_int32),
+        ("chunk_px", ctypes.c_int32),
+        ("scale", ctypes.c_float),
+        ("flip_extent", ctypes.c_float),
+    ]
+
+
 _P = ctypes.POINTER
 _f32p = _P(ctypes.c_float)
 _i32p = _P(ctypes.c_int32)
@@ -132,6 +143,8 @@ SIGNATURES = {
     "mpm_readback": (ctypes.c_int32, [_vp, _f32p, _f32p, _f32p, _f32p, _f32p, _u8p, _i32p]),
     "mpm_read_grid": (ctypes.c_int32, [_vp, _f32p, _f32p, _f32p]),
     "mpm_count": (ctypes.c_int32, [_vp, _i32p]),
+    "mpm_save_state": (ctypes.c_int32, [_vp, ctypes.c_char_p]),
+    "mpm_load_state": (ctypes.c_int32, [_vp, ctypes.c_char_p]),
     "mpm_hose_pipe": (
         ctypes.c_int32,
         [_vp, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
@@ -149,6 +162,13 @@ SIGNATURES = {
     "mpm_last_error": (ctypes.c_int32, [_vp]),
     "mpm_error_name": (ctypes.c_char_p, [ctypes.c_int32]),
     "mpm_last_message": (ctypes.c_char_p, [_vp]),
+    # include/mpm_render.h
+    "mpm_renderer_create": (ctypes.c_int32, [_P(MpmRenderDesc), _P(_vp)]),
+    "mpm_renderer_destroy": (ctypes.c_int32, [_vp]),
+    "mpm_render_frame": (ctypes.c_int32, [_vp, _P(MpmView), _vp, ctypes.c_int32]),
+    "mpm_write_ppm": (ctypes.c_int32, [ctypes.c_char_p, _u8p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+    "mpm_render_last_error": (ctypes.c_int32, []),
+    "mpm_render_message": (ctypes.c_char_p, []),
     # include/mpm_diag.h
     "mpm_debug_bin": (ctypes.c_int32, [_vp, _P(ctypes.c_uint32), _i32p, _i32p]),
     "mpm_time_kernels": (ctypes.c_int32, [_vp, ctypes.c_int32, _f32p, _i32p]),

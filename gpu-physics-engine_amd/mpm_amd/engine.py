@@ -7,7 +7,8 @@ Reference interface (libs/cuda-engine/include/cuda-engine/cuda-engine.h) and its
   cudaEventQuery(tick_done)                 -> Engine.query()             (mpm_query)
   cudaEventElapsedTime(t0, t1)              -> Engine.last_step_ms()      (mpm_last_step_ms)
   OutputBuffer::readTarget()                -> Engine.read_view()         (mpm_read_view)
-plus host readbacks for tests (Engine.readback / Engine.read_grid).  Every call raises MpmError on a
+plus host readbacks for tests (Engine.readback / Engine.read_grid) and state files (Engine.save_state /
+Engine.load_state).  Every call raises MpmError on a
 negative status; there is no fallback path.
 """
 import ctypes
@@ -154,6 +155,14 @@ class Engine:
         view = abi.MpmView()
         _check(self.lib, self.h, self.lib.mpm_read_view(self.h, ctypes.byref(view)), "mpm_read_view")
         return view
+
+    def save_state(self, path):
+        """Write the particle state to a state file (mpm_save_state)."""
+        _check(self.lib, self.h, self.lib.mpm_save_state(self.h, str(path).encode()), "mpm_save_state")
+
+    def load_state(self, path):
+        """Replace the population with the one in a state file (mpm_load_state)."""
+        _check(self.lib, self.h, self.lib.mpm_load_state(self.h, str(path).encode()), "mpm_load_state")
 
     # -- diagnostics (include/mpm_diag.h) --------------------------------------------------------------
     def debug_bin(self):

@@ -10,7 +10,8 @@
  *                                                                        -> mpm_step / mpm_query / mpm_sync /
  *                                                                           mpm_last_step_ms
  *   - OutputBuffer / View / MPMTickStaging (cuda-engine.h:132-161, 312-342; MPM.cu:19-44, 302-322)
- *                                                                        -> mpm_read_view / mpm_roll_output
+ *                                                                        -> mpm_read_view (each tick rolls the
+ *                                                                           slots on the device stream)
  *   - hosePipe         (cuda-engine.h:298-307, MPM.cu:439-473)            -> mpm_hose_pipe
  * and follows the error convention of the reference's only C-ABI (libs/render/include/render/cuda-interface.h:30-42,
  * libs/render/source/cuda-render.cu:23-27): int32_t status, 0 = ok, negative = error, plus last_error /
@@ -150,6 +151,19 @@ int32_t mpm_readback(MpmEngine* h, float* x, float* v, float* C, float* F, float
 int32_t mpm_read_grid(MpmEngine* h, float* mass, float* vel_x, float* vel_y);
 /* Current particle count (device-resident count slot 0, cuda-engine.h:287-289). */
 int32_t mpm_count(MpmEngine* h, int32_t* n);
+
+/* ---- state file (checkpoint / resume; the reference has none) ----
+ * Little-endian binary, a 200-byte header {char magic[8] = "MPMSTATE"; uint32 version = 1; uint32 header_bytes
+ * = 200; int32 n; int32 num_scenes; uint64 checksum (64-bit FNV-1a of the payload); MpmParams params (164 bytes,
+ * the saving handle's); uint32 reserved = 0}, then the payload in caller particle order: x f32[2n] |
+ * v f32[2n] | C f32[4n] | F f32[4n] | J f32[n] | scene i32[n] | material u8[n]. */
+/* Synchronous: write the current state (waits for enqueued work). */
+int32_t mpm_save_state(MpmEngine* h, const char* path);
+/* Replace the population with a saved one; particle i of the file becomes particle i of the handle.  The
+ * file's grid size and scene count must equal the handle's (MPM_ERR_INVALID_ARG otherwise; also for a bad
+ * magic, version, size or checksum); n above capacity gives MPM_ERR_CAPACITY.  The handle keeps its own
+ * physical parameters. */
+int32_t mpm_load_state(MpmEngine* h, const char* path);
 
 /* Hose-pipe emitter (hosePipe, MPM.cu:439-473): enqueue width*rows particles on the handle's stream in
  * scene 0.  origin/velocity in scene units, velocity is the per-tick displacement (divided by the

@@ -52,6 +52,8 @@ def lib():
         L.oracle_mlp_loss_grad.restype = ctypes.c_double
         L.oracle_adam.argtypes = [ctypes.c_int32, _d, _d, _d, _d, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                   ctypes.c_double, ctypes.c_int32]
+        L.oracle_render.argtypes = [ctypes.c_int32, _f32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                    ctypes.c_float, ctypes.c_float, _u8]
         _lib = L
     return _lib
 
@@ -212,3 +214,12 @@ class MLP:
 def adam(w, g, m, v, lr, beta1, beta2, eps, t):
     """One Adam step in place on flat float64 arrays."""
     lib().oracle_adam(w.shape[0], _p(w), _p(g), _p(m), _p(v), lr, beta1, beta2, eps, int(t))
+
+
+def render(pos, width, height, chunk_px=160, scale=5.0, flip_extent=0.0):
+    """Serial frame of the reference renderer: background, then every particle blended in index order.
+    pos: float32 [n,2]; returns uint8 [height, width, 4]."""
+    pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 2)
+    img = np.zeros((height, width, 4), np.uint8)
+    lib().oracle_render(pos.shape[0], _p(pos, _f32), width, height, chunk_px, scale, flip_extent, _p(img, _u8))
+    return img

@@ -32,6 +32,8 @@
 // counting sort must reproduce.
 //
 // MLP restatement: forward, backward and Adam of the 4-64-64-3 SiLU closure in FP64, for gradient checks.
+//
+// Frame restatement: the reference renderer's background and particle blend, serial (oracle_render).
 #include "../include/mpm_engine.h"
 
 #include <algorithm>
@@ -610,6 +612,58 @@ void oracle_adam(int32_t n, double* w, const double* g, double* m, double* v, do
         v[i] = beta2 * v[i] + (1.0 - beta2) * g[i] * g[i];
         const double mh = m[i] / c1, vh = v[i] / c2;
         w[i] -= lr * mh / (std::sqrt(vh) + eps);
+    }
+}
+
+// Serial restatement of the reference's frame: backgroundKernel (libs/render/source/cuda-render.cu:214-243)
+// over the whole image, then drawKernel (:259-310) for particles 0..n-1 in index order, each blending the
+// colour (140, 210, 255, alpha 100) with alpha() (:244-257) over the 5x5 block minus corners around
+// ((int)(x * scale), (int)(y * scale)).  FP32 arithmetic as written (this file is built without contraction).
+// flip_extent > 0 draws y as flip_extent - y.  rgba: uint8[4 * width * height], rows packed.
+void oracle_render(int32_t n, const float* pos, int32_t width, int32_t height, int32_t chunk_px, float scale,
+                   float flip_extent, uint8_t* rgba)
+{
+    for (int y = 0; y < height; ++y) {
+        for (int x = 0; x < width; ++x) {
+            uint8_t* px = rgba + 4 * ((size_t)y * width + x);
+            const bool line = chunk_px > 0 && (x % chunk_px == 0 || y % chunk_px == 0);
+            if (line) {
+                px[0] = 0;
+                px[1] = 0;
+                px[2] = 0;
+            } else {
+                const uint8_t shade = (uint8_t)(100.f * ((float)y / (float)height));
+                px[0] = (uint8_t)(255u - shade);
+                px[1] = (uint8_t)(160u - shade);
+                px[2] = 80;
+            }
+            px[3] = 255;
+        }
+    }
+    const float c = 1.f / 255.f;
+    const uint8_t over[4] = {140, 210, 255, 100};
+    const float a = (float)over[3] * c;
+    const float ia = 1.f - a;
+    for (int i = 0; i < n; ++i) {
+        const float fx = pos[2 * i];
+        float fy = pos[2 * i + 1];
+        if (flip_extent > 0.f) { fy = flip_extent - fy; }
+        const float sx = fx * scale, sy = fy * scale;
+        if (!(sx > -2147483520.f && sx < 2147483520.f && sy > -2147483520.f && sy < 2147483520.f)) { continue; }
+        const int32_t rx = (int32_t)sx, ry = (int32_t)sy;
+        if ((size_t)(int64_t)rx >= (size_t)width || (size_t)(int64_t)ry >= (size_t)height) { continue; }
+        for (int di = -2; di <= 2; ++di) {
+            if (ry + di >= height || ry + di < 0) { continue; }
+            for (int dj = -2; dj <= 2; ++dj) {
+                if (rx + dj >= width || rx + dj < 0) { continue; }
+                if (std::abs(di) == 2 && std::abs(dj) == 2) { continue; }
+                uint8_t* px = rgba + 4 * ((size_t)(ry + di) * width + (rx + dj));
+                for (int ch = 0; ch < 3; ++ch) {
+                    px[ch] = (uint8_t)((float)over[ch] * a + (float)px[ch] * ia + 0.5f);
+                }
+                px[3] = 255;
+            }
+        }
     }
 }
 

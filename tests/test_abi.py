@@ -14,7 +14,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared_symbols():
     names = []
-    for h in ("mpm_engine.h", "mpm_diag.h"):
+    for h in ("mpm_engine.h", "mpm_diag.h", "mpm_render.h"):
         txt = open(os.path.join(ROOT, "include", h)).read()
         txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
         names += re.findall(r"^\s*(?:const\s+)?[\w\*\s]+?\b(mpm_\w+)\s*\(", txt, flags=re.M)

@@ -244,6 +244,12 @@ def test_output_view_and_emitter():
         # second emission uses the committed count as base
         e.hose_pipe((30.0, 60.0), (12.0, 0.0), 30.0, 1.26, 0.252, 1)
         assert e.count() == 480
+        # the next slot's view counts an emission without any host query in between
+        e.hose_pipe((30.0, 90.0), (12.0, 0.0), 30.0, 1.26, 0.252, 2)
+        e.step(1)
+        e.sync()
+        view = e.read_view()
+        assert view.slot == 1 and view.count == 720
 
 
 def test_capacity_is_respected():
@@ -253,9 +259,10 @@ def test_capacity_is_respected():
         e.set_particles(np.zeros((0, 2), np.float32))
         e.hose_pipe((30.0, 30.0), (12.0, 0.0), 30.0, 1.26, 0.0, 0)
         e.hose_pipe((30.0, 60.0), (12.0, 0.0), 30.0, 1.26, 0.0, 1)
-        assert e.count() == 300
         e.step(1)
         e.sync()
+        assert e.read_view().count == 300
+        assert e.count() == 300
         assert np.all(np.isfinite(e.readback()["x"]))
 
 
