@@ -1,10 +1,10 @@
 // engine.cpp — host side of the MI355X MLS-MPM engine behind the C-ABI of include/mpm_engine.h.
 //
 // Reference counterparts (libs/cuda-engine):
-//   MPMArena slab + carve (cuda-engine.cpp:100-117, 234-263) -> Engine::allocate (one hipMalloc, 256-B carve)
-//   GraphEnv (cuda-engine.h:97-109, cuda-engine.cpp:121-132)  -> Engine stream / events / graph execs
+//   MPMArena slab + carve (cuda-engine.cpp:146-174)         -> Engine::allocate (one hipMalloc, 256-B carve)
+//   GraphEnv (cuda-engine.h:16-28, cuda-engine.cpp:33-44)    -> Engine stream / events / graph execs
 //   instantiateMPMGraph (MPM.cu:326-374)                      -> Engine::build_graphs (stream capture)
-//   OutputBuffer roll/readTarget/writeTarget (cuda-engine.cpp:160-186) -> write_slot / read_slot below
+//   OutputBuffer roll/readTarget/writeTarget (cuda-engine.cpp:72-98) -> write_slot / read_slot below
 //   MPMTickStaging (MPM.cu:19-44)                             -> pinned OutView per slot + hipMemcpyToSymbolAsync
 //   hosePipe (MPM.cu:439-473)                                 -> mpm_hose_pipe
 #include <hip/hip_runtime.h>
@@ -39,7 +39,7 @@ struct MpmEngine {
     Store store[2]{};
     Grid grid{};
     SortScratch ss{};
-    int32_t* count = nullptr;  // device [2]: slot 0 authoritative, slot 1 staged (cuda-engine.h:287-289)
+    int32_t* count = nullptr;  // device [2]: slot 0 authoritative, slot 1 staged (cuda-engine.h:206-208)
     float* out_pos[3]{};
     float* out_vel[3]{};
     int32_t* out_num[3]{};

@@ -1,7 +1,7 @@
 // render.hip — headless particle rasteriser behind include/mpm_render.h.
 //
 // Reference counterparts (libs/render/source/cuda-render.cu): backgroundKernel (:214-243), drawKernel and its
-// alpha blend (:244-310), launchFill (:188-212), cuda_engine_render_frame (:163-186).  The reference blends
+// alpha blend (:244-310), launchFill (:189-213), cuda_engine_render_frame (:163-186).  The reference blends
 // every particle into the image with an unsynchronised read-modify-write; here three passes make the same
 // image deterministically:
 //   1. k_render_clear     zero the per-pixel coverage counters
@@ -21,8 +21,8 @@
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kColour[3] = {140, 210, 255};  // drawKernel's particle colour (cuda-render.cu:293-296)
-constexpr int kAlpha = 100;                  // and alpha (cuda-render.cu:297)
+constexpr int kColour[3] = {140, 210, 255};  // drawKernel's particle colour (cuda-render.cu:292-294)
+constexpr int kAlpha = 100;                  // and alpha (cuda-render.cu:295)
 
 struct BlendLut {
     uint8_t c[3][256];

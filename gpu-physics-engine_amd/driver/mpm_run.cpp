@@ -2,8 +2,8 @@
 //
 // Counterpart of the reference's tick loop (main/MPM-master.cpp:33-125) without the window and GUI thread:
 //   - hose-pipe emission every tick until TARGET_COUNT particles (MPM-master.cpp:19-25, 103-105),
-//   - one engine tick per loop iteration, timed on the device (t0 / t1 events, MPM-master.cpp:106-109, 96-98),
-//   - every 150 ticks the average and longest tick time and the particle count (MPM-master.cpp:88-95),
+//   - one engine tick per loop iteration, timed on the device (t0 / t1 events, MPM-master.cpp:106-109, 90-92),
+//   - every 150 ticks the average and longest tick time and the particle count (MPM-master.cpp:94-102),
 //   - optionally the frame of the reference renderer written as PPM images (cuda-render.cu:163-310).
 // The reference paces ticks at 32 ms of wall time (kTickPeriod, constants.h:24); --realtime does the same,
 // the default runs ticks back to back.

@@ -2,7 +2,7 @@
  * mpm_diag.h — diagnostic entry points of the MI355X MLS-MPM engine.
  *
  * None of these has a counterpart in the reference engine (libs/cuda-engine exposes no test or timing
- * hooks; its only instrument is the t0/t1 event pair around a whole tick, main/MPM-master.cpp:368-371).
+ * hooks; its only instrument is the t0/t1 event pair around a whole tick, main/MPM-master.cpp:106-109).
  * They exist for the parity tests (bin order, stress) and for per-kernel timing with HIP events on the
  * engine's own stream.  All are synchronous and use the handle's stream.
  */

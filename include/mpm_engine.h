@@ -3,16 +3,16 @@
  *
  * This header replaces the C++ surface of the reference engine
  * (libs/cuda-engine/include/cuda-engine/cuda-engine.h) for the MPM path:
- *   - MPMArena         (cuda-engine.h:271-297, cuda-engine.cpp:234-263)  -> mpm_create / mpm_set_particles
- *   - GraphEnv         (cuda-engine.h:97-109)                            -> owned by the handle
- *   - instantiateMPMGraph (cuda-engine.h:308-311, MPM.cu:326-374)        -> mpm_create (graph built inside)
- *   - cudaGraphLaunch + t0/t1/tick_done events (main/MPM-master.cpp:368-371)
+ *   - MPMArena         (cuda-engine.h:190-216, cuda-engine.cpp:146-174)  -> mpm_create / mpm_set_particles
+ *   - GraphEnv         (cuda-engine.h:16-28)                             -> owned by the handle
+ *   - instantiateMPMGraph (cuda-engine.h:227-230, MPM.cu:326-374)        -> mpm_create (graph built inside)
+ *   - cudaGraphLaunch + t0/t1/tick_done events (main/MPM-master.cpp:106-109)
  *                                                                        -> mpm_step / mpm_query / mpm_sync /
  *                                                                           mpm_last_step_ms
- *   - OutputBuffer / View / MPMTickStaging (cuda-engine.h:132-161, 312-342; MPM.cu:19-44, 302-322)
+ *   - OutputBuffer / View / MPMTickStaging (cuda-engine.h:51-80, 231-260; MPM.cu:19-44, 302-322)
  *                                                                        -> mpm_read_view (each tick rolls the
  *                                                                           slots on the device stream)
- *   - hosePipe         (cuda-engine.h:298-307, MPM.cu:439-473)            -> mpm_hose_pipe
+ *   - hosePipe         (cuda-engine.h:217-226, MPM.cu:439-473)            -> mpm_hose_pipe
  * and follows the error convention of the reference's only C-ABI (libs/render/include/render/cuda-interface.h:30-42,
  * libs/render/source/cuda-render.cu:23-27): int32_t status, 0 = ok, negative = error, plus last_error /
  * error_name queries.  No exception or abort crosses this boundary.
@@ -20,11 +20,11 @@
  * All pointers are plain host pointers unless a comment says "device pointer".  Vector fields use the
  * reference's interleaved layout at the boundary: positions and velocities as float[2*n] (x0,y0,x1,y1,...)
  * like float2[], and C / F as float[4*n] row-major (F00,F01,F10,F11) like the reference's float4
- * (cuda-engine.h:280-285, cuda-internal.cuh:44-56).  Inside the engine the store is SoA per scalar.
+ * (cuda-engine.h:199-204, cuda-internal.cuh:44-56).  Inside the engine the store is SoA per scalar.
  *
  * Threading: one host thread drives a handle (like the reference's sim thread).  mpm_read_view may be
  * called from a second thread (the reference's renderer thread reads OutputBuffer::readTarget,
- * cuda-engine.cpp:166-175).  Several handles may coexist; batched scenes live inside one handle.
+ * cuda-engine.cpp:78-87).  Several handles may coexist; batched scenes live inside one handle.
  */
 #ifndef MPM_ENGINE_H
 #define MPM_ENGINE_H
@@ -37,7 +37,7 @@ extern "C" {
 
 /* ---- status codes (reference render C-ABI uses 0 / -1 / -2 / -4 / -5, cuda-render.cu:23-60) ---- */
 #define MPM_OK 0
-#define MPM_BUSY 1                  /* mpm_query: work still in flight (cudaErrorNotReady, MPM-master.cpp:345) */
+#define MPM_BUSY 1                  /* mpm_query: work still in flight (cudaErrorNotReady, MPM-master.cpp:83-84) */
 #define MPM_ERR_INVALID_ARG (-1)    /* null handle/pointer, bad size, parameter out of range               */
 #define MPM_ERR_DEVICE (-2)         /* a HIP runtime call failed; mpm_last_error holds the hipError_t       */
 #define MPM_ERR_CAPACITY (-3)       /* particle count above the handle's capacity                            */
@@ -97,7 +97,7 @@ typedef struct MpmParams {
     int32_t reserved[7];
 } MpmParams;
 
-/* Device pointers of one output slot (reference View, cuda-engine.h:132-138). */
+/* Device pointers of one output slot (reference View, cuda-engine.h:51-57). */
 typedef struct MpmView {
     float* pos;         /* device pointer, float[2*capacity], particle i at pos[2i], pos[2i+1]             */
     float* vel;         /* device pointer, float[2*capacity]                                             */
@@ -112,10 +112,10 @@ typedef struct MpmEngine MpmEngine;
 int32_t mpm_params_preset(int32_t preset, MpmParams* out);
 
 /* Allocate the SoA store, grids and output triple buffer on the current device, and build the tick
- * hipGraph.  Replaces MPMArena() (cuda-engine.cpp:234-259), OutputBuffer (cuda-engine.h:143-161),
+ * hipGraph.  Replaces MPMArena() (cuda-engine.cpp:146-171), OutputBuffer (cuda-engine.h:62-80),
  * instantiateMPMGraph (MPM.cu:326-374) and MPMTickStaging::init (MPM.cu:19-30). */
 int32_t mpm_create(const MpmParams* params, MpmEngine** out);
-/* Release everything the handle owns (~MPMArena, ~GraphEnv, ~OutputArena: cuda-engine.cpp:121-132, 151-154, 260-263). */
+/* Release everything the handle owns (~MPMArena, ~GraphEnv, ~OutputArena: cuda-engine.cpp:172-174, 33-44, 63-66). */
 int32_t mpm_destroy(MpmEngine* h);
 
 /*
@@ -129,16 +129,16 @@ int32_t mpm_set_particles(MpmEngine* h, int32_t n, const float* x, const float* 
                           const float* F, const float* J, const uint8_t* material, const int32_t* scene_counts);
 
 /* Enqueue n_ticks ticks of params.substeps substeps each, asynchronously (one graph launch per tick,
- * MPM-master.cpp:368-371).  Each tick ends with the output copy into the next output slot. */
+ * MPM-master.cpp:106-109).  Each tick ends with the output copy into the next output slot. */
 int32_t mpm_step(MpmEngine* h, int32_t n_ticks);
-/* MPM_OK when all enqueued work finished, MPM_BUSY while in flight (cudaEventQuery, MPM-master.cpp:345). */
+/* MPM_OK when all enqueued work finished, MPM_BUSY while in flight (cudaEventQuery, MPM-master.cpp:83-84). */
 int32_t mpm_query(MpmEngine* h);
 /* Block until all enqueued work finished. */
 int32_t mpm_sync(MpmEngine* h);
-/* GPU time of the last tick in ms (cudaEventElapsedTime(t0, t1), MPM-master.cpp:352-354). */
+/* GPU time of the last tick in ms (cudaEventElapsedTime(t0, t1), MPM-master.cpp:90-92). */
 int32_t mpm_last_step_ms(MpmEngine* h, float* ms);
 
-/* Device pointers of the slot most recently completed by a tick (OutputBuffer::readTarget, cuda-engine.cpp:166-175).
+/* Device pointers of the slot most recently completed by a tick (OutputBuffer::readTarget, cuda-engine.cpp:78-87).
  * Safe to call from a second thread. */
 int32_t mpm_read_view(MpmEngine* h, MpmView* out);
 
@@ -149,7 +149,7 @@ int32_t mpm_readback(MpmEngine* h, float* x, float* v, float* C, float* F, float
 /* Synchronous host copy of the grid of the last substep: mass and velocity per node, float[num_scenes*grid_w*grid_h]
  * each, node (scene, y, x) at scene*grid_w*grid_h + y*grid_w + x (reference node index, MPM.cu:122). */
 int32_t mpm_read_grid(MpmEngine* h, float* mass, float* vel_x, float* vel_y);
-/* Current particle count (device-resident count slot 0, cuda-engine.h:287-289). */
+/* Current particle count (device-resident count slot 0, cuda-engine.h:206-208). */
 int32_t mpm_count(MpmEngine* h, int32_t* n);
 
 /* ---- state file (checkpoint / resume; the reference has none) ----

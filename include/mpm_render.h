@@ -6,7 +6,7 @@
  * (cuda-engine.h:51-57) into a swap-chain buffer shared through external memory and semaphores
  * (cuda-render.cu:29-156); here the target is a plain RGBA8 image in device or host memory.
  *   - cuda_engine_init / cuda_engine_bind_buffers (cuda-render.cu:29-156) -> mpm_renderer_create
- *   - cuda_engine_render_frame + launchFill   (cuda-render.cu:163-212)     -> mpm_render_frame
+ *   - cuda_engine_render_frame + launchFill   (cuda-render.cu:163-213)     -> mpm_render_frame
  *   - backgroundKernel (cuda-render.cu:214-243): vertical gradient, black chunk lines
  *   - drawKernel (cuda-render.cu:259-310): every particle blends colour (140, 210, 255) at alpha 100/255
  *     over the 5x5 pixel block around (int)(x * scale), (int)(y * scale), corners excluded
