@@ -28,9 +28,11 @@ HBM_PEAK_GBS = 8000.0
 
 # Algorithmic bytes per unit of the fused G2P->P2G kernel (k_g2p2g):
 #   per particle: read x,y 8 + F 16 + J 4 + material 1 = 29 B, write x,y 8 + F 16 + J 4 = 28 B -> 57 B
-#   per grid node: read velocity 8 B (G2P of substep s) + accumulate mass/momentum 12 B (P2G of s+1) -> 20 B
+#   per grid node: read the previous substep's mass/momentum 12 B (velocities are computed in the kernel: the
+#   grid update is folded in) + accumulate mass/momentum 12 B + zero the set of the substep after next 12 B
+#   -> 36 B
 G2P2G_BYTES_PER_PARTICLE = 57
-G2P2G_BYTES_PER_NODE = 20
+G2P2G_BYTES_PER_NODE = 36
 # Whole-substep algorithmic bytes of the unfused three-kernel form: 132 B per particle, 40 B per node.
 SUBSTEP_BYTES_PER_PARTICLE = 132
 SUBSTEP_BYTES_PER_NODE = 40

@@ -38,6 +38,8 @@ struct MpmEngine {
     size_t slab_bytes = 0;
     Store store[2]{};
     Grid grid{};
+    // Fused mode: three sets of P2G accumulators (set 0 is grid.mom_x/mom_y/mass), all zero between segments.
+    float* acc[3][3]{};
     SortScratch ss{};
     int32_t* count = nullptr;  // device [2]: slot 0 authoritative, slot 1 staged (cuda-engine.h:206-208)
     float* out_pos[3]{};
@@ -209,6 +211,12 @@ int32_t allocate(MpmEngine* h)
         h->grid.mass = c.take<float>(nodes);
         h->grid.vel = c.take<float2>(nodes);
         h->grid.vmass = c.take<float>(nodes);
+        h->acc[0][0] = h->grid.mom_x;
+        h->acc[0][1] = h->grid.mom_y;
+        h->acc[0][2] = h->grid.mass;
+        for (int b = 1; b < 3; ++b) {
+            for (int k = 0; k < 3; ++k) { h->acc[b][k] = p.mode == MPM_MODE_FUSED ? c.take<float>(nodes) : nullptr; }
+        }
         ss.keys_tmp = c.take<uint32_t>(cap);
         ss.keys_alt = c.take<uint32_t>(cap);
         ss.vals_tmp = c.take<int32_t>(cap);
@@ -276,20 +284,39 @@ struct LaunchTimer {
     }
 };
 
+// Grid view of accumulator set `target` (scattered into), optionally computing velocities from set `src`
+// and zeroing set `zero` (fused substep); -1 = none.
+Grid grid_for(const MpmEngine* h, int target, int src, int zero)
+{
+    Grid g = h->grid;
+    g.mom_x = h->acc[target][0];
+    g.mom_y = h->acc[target][1];
+    g.mass = h->acc[target][2];
+    g.src_x = src >= 0 ? h->acc[src][0] : nullptr;
+    g.src_y = src >= 0 ? h->acc[src][1] : nullptr;
+    g.src_m = src >= 0 ? h->acc[src][2] : nullptr;
+    g.zero_x = zero >= 0 ? h->acc[zero][0] : nullptr;
+    g.zero_y = zero >= 0 ? h->acc[zero][1] : nullptr;
+    g.zero_m = zero >= 0 ? h->acc[zero][2] : nullptr;
+    g.zero_n = zero >= 0 ? (size_t)h->dp.G * (size_t)h->params.num_scenes : 0;
+    return g;
+}
+
 // Enqueue `substeps` substeps on the stream starting from store c (c is updated to the store holding the
-// result), then the output copy when `output` is set.
-hipError_t enqueue_substeps(MpmEngine* h, int& c, int substeps, bool output, LaunchTimer* tm)
+// result), then the output copy when `output` is set.  dirty: which accumulator sets hold data, before (in,
+// all zero) and after (out); every segment checks it scatters only into zero sets.
+hipError_t enqueue_substeps(MpmEngine* h, int& c, int substeps, bool output, LaunchTimer* tm, bool dirty[3])
 {
     const MpmParams& p = h->params;
     const DevParams& d = h->dp;
     hipStream_t st = h->stream;
     const int cap = p.capacity;
     hipError_t e = hipSuccess;
-#define ENQ(k, x)                                        \
-    do {                                                 \
+#define ENQ(k, x)                                                 \
+    do {                                                          \
         if (tm && (e = tm->begin(k)) != hipSuccess) { return e; } \
-        e = (x);                                         \
-        if (e != hipSuccess) { return e; }               \
+        e = (x);                                                  \
+        if (e != hipSuccess) { return e; }                        \
         if (tm && (e = tm->end()) != hipSuccess) { return e; }    \
     } while (0)
     if (p.mode == MPM_MODE_REFERENCE) {
@@ -314,12 +341,28 @@ hipError_t enqueue_substeps(MpmEngine* h, int& c, int substeps, bool output, Lau
                     ENQ(MPM_KT_G2P, launch_g2p_tiled(st, h->shape, d, s, h->grid, h->ss, h->count));
                 }
             } else {
-                ENQ(MPM_KT_P2G, launch_p2g_tiled(st, h->shape, d, s, h->grid, h->ss, h->count));
-                for (int k = 0; k < L - 1; ++k) {
-                    ENQ(MPM_KT_GRID, launch_grid_update(st, d, h->grid));
-                    ENQ(MPM_KT_G2P2G, launch_g2p2g(st, h->shape, d, s, h->grid, h->ss, h->count));
+                // Fused: every segment starts and ends with all three accumulator sets zero.  P2G scatters
+                // into set 0; substep k of the segment computes node velocities from set (k-1)%3, scatters
+                // into set k%3 and zeroes set (k+1)%3 (read by the launch before, written by the launch
+                // after); the closing grid update reads the last set, zeroes it and the one before.
+                if (dirty[0] || dirty[1] || dirty[2]) { return hipErrorUnknown; }  // plan invariant
+                int t = 0, prev = -1;
+                ENQ(MPM_KT_P2G, launch_p2g_tiled(st, h->shape, d, s, grid_for(h, t, -1, -1), h->ss, h->count));
+                dirty[t] = true;
+                for (int k = 1; k < L; ++k) {
+                    const int src = t;
+                    t = k % 3;
+                    const int z = (k + 1) % 3;
+                    if (dirty[t]) { return hipErrorUnknown; }
+                    ENQ(MPM_KT_G2P2G, launch_g2p2g(st, h->shape, d, s, grid_for(h, t, src, dirty[z] ? z : -1), h->ss,
+                                                   h->count));
+                    dirty[t] = true;
+                    dirty[z] = false;
+                    prev = src;
                 }
-                ENQ(MPM_KT_GRID, launch_grid_update(st, d, h->grid));
+                ENQ(MPM_KT_GRID, launch_grid_update(st, d, grid_for(h, t, -1, prev)));
+                dirty[t] = false;
+                if (prev >= 0) { dirty[prev] = false; }
                 ENQ(MPM_KT_G2P, launch_g2p_tiled(st, h->shape, d, s, h->grid, h->ss, h->count));
             }
             done += L;
@@ -339,7 +382,8 @@ int32_t build_graphs(MpmEngine* h)
     for (int c = 0; c < 2; ++c) {
         HIP_TRY(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
         int cc = c;
-        hipError_t e = enqueue_substeps(h, cc, p.substeps, true, nullptr);
+        bool dirty[3] = {false, false, false};
+        hipError_t e = enqueue_substeps(h, cc, p.substeps, true, nullptr, dirty);
         hipGraph_t g = nullptr;
         hipError_t e2 = hipStreamEndCapture(h->stream, &g);
         if (e != hipSuccess) {
@@ -1044,7 +1088,8 @@ int32_t mpm_time_kernels(MpmEngine* h, int32_t substeps, float* ms, int32_t* lau
     LaunchTimer tm;
     tm.st = h->stream;
     int c = h->cur;
-    const hipError_t e = enqueue_substeps(h, c, substeps, false, &tm);
+    bool dirty[3] = {false, false, false};
+    const hipError_t e = enqueue_substeps(h, c, substeps, false, &tm, dirty);
     if (e != hipSuccess) { return fail(h, MPM_ERR_DEVICE, (int32_t)e, "timed launch"); }
     h->cur = c;
     HIP_TRY(h, hipStreamSynchronize(h->stream));

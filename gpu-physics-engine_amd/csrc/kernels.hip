@@ -63,7 +63,8 @@ __device__ __forceinline__ void atomic_add_global(float* a, float v) { unsafeAto
 // ===================================================================================== grid update
 
 // gridKernel (MPM.cu:131-177) for every node of every scene, plus the reset of the accumulators it
-// consumed (resetGridKernel, MPM.cu:46-51), so the next P2G starts from zero without a separate pass.
+// consumed (resetGridKernel, MPM.cu:46-51), so the next P2G starts from zero without a separate pass; in
+// the fused substep it also zeroes the accumulator set of the substep before (g.zero_*).
 // Each lane handles 4 consecutive nodes (float4 loads of the accumulator planes).
 __global__ __launch_bounds__(kThreads) void k_grid_update(DevParams p, Grid g, size_t total)
 {
@@ -94,6 +95,11 @@ __global__ __launch_bounds__(kThreads) void k_grid_update(DevParams p, Grid g, s
         *reinterpret_cast<float4*>(g.mom_x + i0) = z;
         *reinterpret_cast<float4*>(g.mom_y + i0) = z;
         *reinterpret_cast<float4*>(g.mass + i0) = z;
+        if (g.zero_x) {
+            *reinterpret_cast<float4*>(g.zero_x + i0) = z;
+            *reinterpret_cast<float4*>(g.zero_y + i0) = z;
+            *reinterpret_cast<float4*>(g.zero_m + i0) = z;
+        }
     } else {
         for (size_t idx = i0; idx < total; ++idx) {
             const int local = (int)(idx % (size_t)p.G);
@@ -104,6 +110,11 @@ __global__ __launch_bounds__(kThreads) void k_grid_update(DevParams p, Grid g, s
             g.mom_x[idx] = 0.f;
             g.mom_y[idx] = 0.f;
             g.mass[idx] = 0.f;
+            if (g.zero_x) {
+                g.zero_x[idx] = 0.f;
+                g.zero_y[idx] = 0.f;
+                g.zero_m[idx] = 0.f;
+            }
         }
     }
 }
@@ -592,6 +603,15 @@ __device__ __forceinline__ void item_origin(const DevParams& p, int key, int BX,
     oy = by * BY - kMargin;
 }
 
+// Velocity of grid node (gx, gy) of the scene at gb: the grid-update output, or, in the fused substep,
+// computed here from the previous substep's accumulators (gridKernel, MPM.cu:131-177, folded into G2P).
+__device__ __forceinline__ float2 grid_velocity(const DevParams& p, const Grid& g, size_t gb, int gx, int gy)
+{
+    const size_t n = gb + (size_t)gy * p.W + gx;
+    if (g.src_m) { return node_velocity(p, g.src_x[n], g.src_y[n], g.src_m[n], gx, gy); }
+    return g.vel[n];
+}
+
 template <int BX, int BY>
 __device__ __forceinline__ void load_vel_tile(const DevParams& p, const Grid& g, size_t gb, int ox, int oy, float2* sv)
 {
@@ -601,8 +621,22 @@ __device__ __forceinline__ void load_vel_tile(const DevParams& p, const Grid& g,
         const int tx = t - ty * T::TX;
         const int gx = ox + tx, gy = oy + ty;
         float2 v = {0.f, 0.f};
-        if (gx >= 0 && gx < p.W && gy >= 0 && gy < p.H) { v = g.vel[gb + (size_t)gy * p.W + gx]; }
+        if (gx >= 0 && gx < p.W && gy >= 0 && gy < p.H) { v = grid_velocity(p, g, gb, gx, gy); }
         sv[t] = v;
+    }
+}
+
+// Fused substep: every work group of the launch (also those without a work item) zeroes a slice of the
+// accumulators the substep after next scatters into.  Nobody reads or writes them during this launch: the
+// previous launch read them, the next one writes the other buffer.
+__device__ __forceinline__ void zero_next_accumulators(const Grid& g)
+{
+    if (!g.zero_x) { return; }
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < g.zero_n; i += stride) {
+        g.zero_x[i] = 0.f;
+        g.zero_y[i] = 0.f;
+        g.zero_m[i] = 0.f;
     }
 }
 
@@ -613,7 +647,7 @@ template <int BX, int BY>
 __device__ __forceinline__ void zero_acc_tiles(float4* acc)
 {
     using T = TileShape<BX, BY>;
-    for (int t = threadIdx.x; t < T::N * (kThreads / 64); t += kThreads) { acc[t] = make_float4(0.f, 0.f, 0.f, 0.f); }
+    for (int t = threadIdx.x; t < T::N * kAccTiles; t += kThreads) { acc[t] = make_float4(0.f, 0.f, 0.f, 0.f); }
 }
 
 // One HBM atomic per touched tile node and component (the LDS tiles already reduced every particle of the
@@ -626,7 +660,7 @@ __device__ __forceinline__ void flush_acc_tiles(const DevParams& p, const Grid& 
     for (int t = threadIdx.x; t < T::N; t += kThreads) {
         float4 v = acc[t];
 #pragma unroll
-        for (int w = 1; w < kThreads / 64; ++w) {
+        for (int w = 1; w < kAccTiles; ++w) {
             const float4 u = acc[w * T::N + t];
             v.x += u.x;
             v.y += u.y;
@@ -679,6 +713,41 @@ __device__ __forceinline__ void gather_moments(const float wx[3], const float wy
     }
 }
 
+// Gather from HBM for a particle that drifted out of its tile (rare): rolled loops keep its registers
+// from counting against the tiled path.  Same sums as gather_moments, same order.
+__device__ __forceinline__ void gather_moments_hbm(const DevParams& p, const Grid& g, size_t gb, int bx, int by,
+                                                   const float wx[3], const float wy[3], float& sx, float& sy,
+                                                   float& ax, float& ay, float& cx, float& cy)
+{
+    sx = sy = ax = ay = cx = cy = 0.f;
+#pragma unroll 1
+    for (int a = 0; a < 3; ++a) {
+        float ux = 0.f, uy = 0.f, dx = 0.f, dy = 0.f;
+#pragma unroll 1
+        for (int b = 0; b < 3; ++b) {
+            const float2 gv = grid_velocity(p, g, gb, bx + a, by + b);
+            const float w = b == 0 ? wy[0] : (b == 1 ? wy[1] : wy[2]);
+            ux = fmaf(w, gv.x, ux);
+            uy = fmaf(w, gv.y, uy);
+            if (b > 0) {
+                const float wb = w * (float)b;
+                dx = fmaf(wb, gv.x, dx);
+                dy = fmaf(wb, gv.y, dy);
+            }
+        }
+        const float wxa = a == 0 ? wx[0] : (a == 1 ? wx[1] : wx[2]);
+        sx = fmaf(wxa, ux, sx);
+        sy = fmaf(wxa, uy, sy);
+        cx = fmaf(wxa, dx, cx);
+        cy = fmaf(wxa, dy, cy);
+        if (a > 0) {
+            const float wa = wxa * (float)a;
+            ax = fmaf(wa, ux, ax);
+            ay = fmaf(wa, uy, ay);
+        }
+    }
+}
+
 template <int BX, int BY>
 __device__ __forceinline__ void gather(const DevParams& p, const Grid& g, size_t gb, int ox, int oy, const float2* sv,
                                        float px, float py, float& vx, float& vy, Mat2& B)
@@ -697,8 +766,7 @@ __device__ __forceinline__ void gather(const DevParams& p, const Grid& g, size_t
         const float2* t0 = sv + ly * T::TX + lx;
         gather_moments(wx, wy, [&](int a, int b) { return t0[b * T::TX + a]; }, sx, sy, mx, my, nx, ny);
     } else {
-        const float2* g0 = g.vel + gb + (size_t)by * p.W + bx;
-        gather_moments(wx, wy, [&](int a, int b) { return g0[(size_t)b * p.W + a]; }, sx, sy, mx, my, nx, ny);
+        gather_moments_hbm(p, g, gb, bx, by, wx, wy, sx, sy, mx, my, nx, ny);
     }
     vx = sx;
     vy = sy;
@@ -782,13 +850,15 @@ __device__ __forceinline__ void scatter(const DevParams& p, const Grid& g, size_
         peers &= bit ? m : ~m;
     }
     const int rank = __popcll(peers & lt_mask);
-    for (int r = 0;; ++r) {
-        if (inside && rank == r) {
+    // ranks r .. r + kTilesPerWave - 1 update in the same round, each in its own tile of the wave
+    for (int r = 0;; r += kTilesPerWave) {
+        if (inside && rank >= r && rank < r + kTilesPerWave) {
+            float4* tile = wtile + (rank - r) * T::N;
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
 #pragma unroll
                 for (int b = 0; b < 3; ++b) {
-                    float4* cell = &wtile[base + b * T::TX + a];
+                    float4* cell = &tile[base + b * T::TX + a];
                     float4 v = *cell;
                     v.x += cx[a * 3 + b];
                     v.y += cy[a * 3 + b];
@@ -798,7 +868,7 @@ __device__ __forceinline__ void scatter(const DevParams& p, const Grid& g, size_
                 }
             }
         }
-        if (__ballot(inside && rank > r) == 0ull) { break; }
+        if (__ballot(inside && rank >= r + kTilesPerWave) == 0ull) { break; }
     }
 }
 
@@ -808,7 +878,8 @@ __global__ __launch_bounds__(kThreads) void k_p2g_tiled(DevParams p, Store s, Gr
                                                         const int32_t* num_items, const int32_t* count)
 {
     using T = TileShape<BX, BY>;
-    __shared__ float4 acc[T::N * (kThreads / 64)];
+    __shared__ float4 acc[T::N * kAccTiles];
+    zero_next_accumulators(g);
     bool ok;
     const int it = xcd_item(num_items[0], ok);
     if (!ok) { return; }
@@ -819,7 +890,7 @@ __global__ __launch_bounds__(kThreads) void k_p2g_tiled(DevParams p, Store s, Gr
     const size_t gb = (size_t)scene * (size_t)p.G;
     zero_acc_tiles<BX, BY>(acc);
     __syncthreads();
-    float4* wtile = acc + (threadIdx.x >> 6) * T::N;
+    float4* wtile = acc + (threadIdx.x >> 6) * kTilesPerWave * T::N;
     for (int j = item.begin + threadIdx.x; j < item.end; j += kThreads) {
         const uint32_t o = (uint32_t)j * 4u;
         const Mat2 Q = affine_q(p, ldo(s.mat, (uint32_t)j), ld_C(s, o), ld_F(s, o), ldo(s.J, o));
@@ -874,7 +945,8 @@ __global__ __launch_bounds__(kThreads) void k_g2p2g(DevParams p, Store s, Grid g
 {
     using T = TileShape<BX, BY>;
     __shared__ float2 sv[T::N];
-    __shared__ float4 acc[T::N * (kThreads / 64)];
+    __shared__ float4 acc[T::N * kAccTiles];
+    zero_next_accumulators(g);
     bool ok;
     const int it = xcd_item(num_items[0], ok);
     if (!ok) { return; }
@@ -886,7 +958,7 @@ __global__ __launch_bounds__(kThreads) void k_g2p2g(DevParams p, Store s, Grid g
     load_vel_tile<BX, BY>(p, g, gb, ox, oy, sv);
     zero_acc_tiles<BX, BY>(acc);
     __syncthreads();
-    float4* wtile = acc + (threadIdx.x >> 6) * T::N;
+    float4* wtile = acc + (threadIdx.x >> 6) * kTilesPerWave * T::N;
     for (int j = item.begin + threadIdx.x; j < item.end; j += kThreads) {
         const uint32_t o = (uint32_t)j * 4u;
         float px = ldo(s.x, o), py = ldo(s.y, o), vx, vy;
@@ -932,8 +1004,9 @@ __global__ __launch_bounds__(kThreads) void k_p2g_tiled_learned(DevParams p, Sto
                                                                 const int32_t* num_items, const int32_t* count)
 {
     using T = TileShape<BX, BY>;
-    __shared__ float4 acc[T::N * (kThreads / 64)];
+    __shared__ float4 acc[T::N * kAccTiles];
     __shared__ ClosurePacked cl;
+    zero_next_accumulators(g);
     bool ok;
     const int it = xcd_item(num_items[0], ok);
     if (!ok) { return; }
@@ -946,7 +1019,7 @@ __global__ __launch_bounds__(kThreads) void k_p2g_tiled_learned(DevParams p, Sto
     zero_acc_tiles<BX, BY>(acc);
     closure_stage(p.cl, &cl);
     __syncthreads();
-    float4* wtile = acc + (threadIdx.x >> 6) * T::N;
+    float4* wtile = acc + (threadIdx.x >> 6) * kTilesPerWave * T::N;
     for (int base = item.begin; base < item.end; base += kThreads) {
         const int j = base + threadIdx.x;
         const bool valid = j < item.end;
@@ -967,8 +1040,9 @@ __global__ __launch_bounds__(kThreads) void k_g2p2g_learned(DevParams p, Store s
 {
     using T = TileShape<BX, BY>;
     __shared__ float2 sv[T::N];
-    __shared__ float4 acc[T::N * (kThreads / 64)];
+    __shared__ float4 acc[T::N * kAccTiles];
     __shared__ ClosurePacked cl;
+    zero_next_accumulators(g);
     bool ok;
     const int it = xcd_item(num_items[0], ok);
     if (!ok) { return; }
@@ -982,7 +1056,7 @@ __global__ __launch_bounds__(kThreads) void k_g2p2g_learned(DevParams p, Store s
     zero_acc_tiles<BX, BY>(acc);
     closure_stage(p.cl, &cl);
     __syncthreads();
-    float4* wtile = acc + (threadIdx.x >> 6) * T::N;
+    float4* wtile = acc + (threadIdx.x >> 6) * kTilesPerWave * T::N;
     for (int base = item.begin; base < item.end; base += kThreads) {
         const int j = base + threadIdx.x;
         const bool valid = j < item.end;

@@ -19,6 +19,13 @@ constexpr int kRadixBins = 1 << kRadixBits;
 constexpr int kRadixItems = 16;      // keys per thread per sort tile
 constexpr int kRadixTile = kThreads * kRadixItems;
 constexpr int kMargin = 2;           // nodes of drift a binned particle may take before leaving its tile
+#ifndef MPM_TILES_PER_WAVE
+#define MPM_TILES_PER_WAVE 1
+#endif
+// LDS accumulation tiles per wave: lanes whose stencils start at the same node update different tiles in
+// one round, so a wave needs ceil(multiplicity / kTilesPerWave) rounds.
+constexpr int kTilesPerWave = MPM_TILES_PER_WAVE;
+constexpr int kAccTiles = kTilesPerWave * (kThreads / 64);  // tiles per work group
 constexpr int kMaxScenes = 4096;
 
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
@@ -68,11 +75,21 @@ struct Store {
 
 // Grid of all scenes: scene s, node (x, y) at s*G + y*W + x (reference node index, MPM.cu:122).
 struct Grid {
-    float* mom_x;   // P2G accumulators, zero between substeps
+    float* mom_x;   // P2G accumulators this launch scatters into (zero when it starts)
     float* mom_y;
     float* mass;
     float2* vel;    // grid-update output read by G2P
     float* vmass;   // mass of the last grid update (readback)
+    // Fused substep only (null otherwise): the accumulators of the previous substep, from which the
+    // launch computes node velocities itself (the grid update folded into the G2P gather), and the
+    // accumulators it zeroes for the substep after next (nodes per plane: zero_n).
+    const float* src_x;
+    const float* src_y;
+    const float* src_m;
+    float* zero_x;
+    float* zero_y;
+    float* zero_m;
+    size_t zero_n;
 };
 
 // Work item of the binned kernels: particles [begin, end) of bin `key` (all within one bin).
