@@ -261,23 +261,44 @@ int32_t allocate(MpmEngine* h)
 // Optional per-launch HIP event pairs (mpm_time_kernels); absent during graph capture.
 struct LaunchTimer {
     hipStream_t st = nullptr;
-    std::vector<hipEvent_t> ev;
-    std::vector<int> cls;
-    // Timing events without the system-scope release a default event performs (an L2 write-back that
-    // would otherwise be counted inside the bracketed kernel).
+    std::vector<hipEvent_t> ev;  // one (start, end) pair per group
+    std::vector<int> cls;        // class of each group
+    std::vector<int> count;      // launches in each group
+    bool open = false;           // the last group still has no end event
+    // Consecutive launches of one class form a group bracketed by one event pair, so back-to-back runs
+    // (the fused substeps of a segment) carry no event packets between their kernels.  The events skip the
+    // system-scope release a default event performs (an L2 write-back that would otherwise be counted
+    // inside the bracketed kernels).
     hipError_t begin(int k)
     {
+        if (open && cls.back() == k) {
+            ++count.back();
+            return hipSuccess;
+        }
+        hipError_t e = close();
+        if (e != hipSuccess) { return e; }
         hipEvent_t a, b;
-        hipError_t e = hipEventCreateWithFlags(&a, hipEventDisableSystemFence);
+        e = hipEventCreateWithFlags(&a, hipEventDisableSystemFence);
         if (e != hipSuccess) { return e; }
         e = hipEventCreateWithFlags(&b, hipEventDisableSystemFence);
-        if (e != hipSuccess) { return e; }
+        if (e != hipSuccess) {
+            (void)hipEventDestroy(a);
+            return e;
+        }
         ev.push_back(a);
         ev.push_back(b);
         cls.push_back(k);
+        count.push_back(1);
+        open = true;
         return hipEventRecord(a, st);
     }
-    hipError_t end() { return hipEventRecord(ev.back(), st); }
+    hipError_t end() { return hipSuccess; }  // the group ends when another class starts (or at close)
+    hipError_t close()
+    {
+        if (!open) { return hipSuccess; }
+        open = false;
+        return hipEventRecord(ev.back(), st);
+    }
     ~LaunchTimer()
     {
         for (hipEvent_t e : ev) { (void)hipEventDestroy(e); }
@@ -393,6 +414,8 @@ int32_t build_graphs(MpmEngine* h)
         if (e2 != hipSuccess) { return fail(h, MPM_ERR_DEVICE, (int32_t)e2, "hipStreamEndCapture failed"); }
         h->graph[c] = g;
         HIP_TRY(h, hipGraphInstantiate(&h->exec[c], g, nullptr, nullptr, 0));
+        // upload now: the first launch of an exec that is not uploaded leaves gaps between its kernels
+        HIP_TRY(h, hipGraphUpload(h->exec[c], h->stream));
     }
     return MPM_OK;
 }
@@ -1089,7 +1112,8 @@ int32_t mpm_time_kernels(MpmEngine* h, int32_t substeps, float* ms, int32_t* lau
     tm.st = h->stream;
     int c = h->cur;
     bool dirty[3] = {false, false, false};
-    const hipError_t e = enqueue_substeps(h, c, substeps, false, &tm, dirty);
+    hipError_t e = enqueue_substeps(h, c, substeps, false, &tm, dirty);
+    if (e == hipSuccess) { e = tm.close(); }
     if (e != hipSuccess) { return fail(h, MPM_ERR_DEVICE, (int32_t)e, "timed launch"); }
     h->cur = c;
     HIP_TRY(h, hipStreamSynchronize(h->stream));
@@ -1101,7 +1125,7 @@ int32_t mpm_time_kernels(MpmEngine* h, int32_t substeps, float* ms, int32_t* lau
         float t = 0.f;
         HIP_TRY(h, hipEventElapsedTime(&t, tm.ev[2 * i], tm.ev[2 * i + 1]));
         ms[tm.cls[i]] += t;
-        launches[tm.cls[i]] += 1;
+        launches[tm.cls[i]] += tm.count[i];
     }
     return MPM_OK;
 }

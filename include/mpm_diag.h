@@ -32,7 +32,8 @@ extern "C" {
 int32_t mpm_debug_bin(MpmEngine* h, uint32_t* keys, int32_t* ids, int32_t* n);
 
 /* Advance `substeps` substeps with the handle's mode, launching eagerly (not through the tick graph) and
- * bracketing every kernel launch with a pair of HIP events on the engine stream.  ms[k] receives the
+ * bracketing every run of consecutive launches of one class (for example the fused substeps of a
+ * segment, which run back to back) with a pair of HIP events on the engine stream.  ms[k] receives the
  * summed GPU time of class k, launches[k] the number of launches (arrays of MPM_KT_COUNT).  The output
  * slot is not written.  Advances the simulation exactly like mpm_step would. */
 int32_t mpm_time_kernels(MpmEngine* h, int32_t substeps, float* ms, int32_t* launches);
