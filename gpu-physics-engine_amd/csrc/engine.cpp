@@ -120,6 +120,7 @@ void derive(const MpmParams& p, DevParams& d)
     d.nbx = (p.grid_w + p.block_x - 1) / p.block_x;
     d.nby = (p.grid_h + p.block_y - 1) / p.block_y;
     d.bps = d.nbx * d.nby;
+    d.node_bits = ilog2_ceil((uint64_t)p.block_x * (uint64_t)p.block_y);
     d.dx = p.dx;
     d.inv_dx = p.inv_dx;
     d.sim_w = p.sim_w;
@@ -174,7 +175,7 @@ std::string validate(MpmParams& p)
     if (p.chunk < 64 || p.chunk > 65536) { return "chunk out of range"; }
     const int64_t K = (int64_t)p.num_scenes * ((p.grid_w + p.block_x - 1) / p.block_x) *
                       ((p.grid_h + p.block_y - 1) / p.block_y);
-    if (K > (1ll << 30)) { return "too many bins"; }
+    if (K * p.block_x * p.block_y > (1ll << 31)) { return "too many bins"; }
     return "";
 }
 
@@ -188,7 +189,8 @@ int32_t allocate(MpmEngine* h)
     ss.capacity = p.capacity;
     ss.ntiles = (int)((cap + kRadixTile - 1) / kRadixTile);
     ss.num_keys = K;
-    ss.key_bits = std::max(1, ilog2_ceil((uint64_t)K));
+    // sort key: bin (scene-major) above the stencil base node within the bin
+    ss.key_bits = std::max(1, ilog2_ceil((uint64_t)K)) + h->dp.node_bits;
     ss.chunk = p.chunk;
     ss.max_items = (int)((cap + p.chunk - 1) / p.chunk) + K;
     const size_t scan_len = std::max((size_t)kRadixBins * ss.ntiles, (size_t)K);

@@ -279,13 +279,16 @@ hipError_t launch_g2p_reference(hipStream_t st, const DevParams& p, const Store&
 
 // ===================================================================================== binning (stable)
 
-// Bin key of every particle from its position; the scene comes from the previous key.
+// Sort key of every particle from its position: bin key (the scene comes from the previous key) above the
+// stencil base node within the bin.
 __global__ __launch_bounds__(kThreads) void k_bin_keys(DevParams p, Store s, const int32_t* count, uint32_t* keys)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count[0]) { return; }
     const uint32_t scene = p.num_scenes > 1 ? s.key[i] / (uint32_t)p.bps : 0u;
-    keys[i] = bin_key(p, s.x[i], s.y[i], scene);
+    uint32_t node;
+    const uint32_t bin = bin_key(p, s.x[i], s.y[i], scene, node);
+    keys[i] = (bin << p.node_bits) | node;
 }
 
 // Radix upsweep: digit histogram of one tile of kRadixTile keys, written digit-major hist[d * ntiles + t].
@@ -489,17 +492,103 @@ __global__ __launch_bounds__(kThreads) void k_gather(Store src, Store dst, const
     dst.id[j] = src.id[i];
 }
 
-// Bin ranges from the sorted keys (bins not present keep start = end = 0).
+// Bin ranges from the sorted keys, bin = key >> shift (bins not present keep start = end = 0).
 __global__ __launch_bounds__(kThreads) void k_bin_bounds(const uint32_t* keys, const int32_t* count, int nkeys,
-                                                         int32_t* bstart, int32_t* bend)
+                                                         int shift, int32_t* bstart, int32_t* bend)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     const int n = count[0];
     if (j >= n) { return; }
-    const uint32_t k = keys[j];
+    const uint32_t k = keys[j] >> shift;
     if (k >= (uint32_t)nkeys) { return; }
-    if (j == 0 || keys[j - 1] != k) { bstart[k] = j; }
-    if (j == n - 1 || keys[j + 1] != k) { bend[k] = j + 1; }
+    if (j == 0 || (keys[j - 1] >> shift) != k) { bstart[k] = j; }
+    if (j == n - 1 || (keys[j + 1] >> shift) != k) { bend[k] = j + 1; }
+}
+
+// Layered order within each bin.  Input: particles sorted stably by (bin, stencil base node).  Particle j
+// of node n has rank r = j - (first index of n); it goes to layer min(r, kLayers - 1).  Layers follow one
+// another, and inside a layer below the last the particles follow in node order, so the 64 lanes of a wave
+// mostly start their stencils at distinct nodes and the ranked scatter needs fewer rounds.  The last layer
+// takes the remaining particles in (node, rank) order.  One work group per bin.
+constexpr int kLayers = 16;
+constexpr int kMaxBinNodes = 512;  // largest bin (32 x 16 cells)
+__global__ __launch_bounds__(kThreads) void k_layer_bins(const uint32_t* keys_in, const int32_t* perm_in,
+                                                         const int32_t* bstart, const int32_t* bend, int nkeys,
+                                                         int node_bits, uint32_t* keys_out, int32_t* perm_out)
+{
+    __shared__ int st[kMaxBinNodes];                 // first index of each node
+    __shared__ int cnt[kMaxBinNodes];                // particles of each node
+    __shared__ int pre[kLayers][kMaxBinNodes];       // layer < last: nodes before n in the layer; last: particles
+    __shared__ int tot[kLayers][kMaxBinNodes / 64];  // per layer and node wave: nodes (or particles) it holds
+    __shared__ int loff[kLayers];                    // first position of each layer
+    const int k = blockIdx.x;
+    if (k >= nkeys) { return; }
+    const int s = bstart[k], e = bend[k];
+    if (e <= s) { return; }
+    const int nn = 1 << node_bits;
+    const uint32_t mask = (uint32_t)nn - 1u;
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int n = threadIdx.x; n < nn; n += kThreads) {
+        st[n] = 0;
+        cnt[n] = 0;
+    }
+    __syncthreads();
+    for (int j = s + threadIdx.x; j < e; j += kThreads) {
+        const int n = (int)(keys_in[j] & mask);
+        if (j == s || (int)(keys_in[j - 1] & mask) != n) { st[n] = j; }
+        if (j == e - 1 || (int)(keys_in[j + 1] & mask) != n) { cnt[n] = j + 1; }  // one past the run, for now
+    }
+    __syncthreads();
+    for (int n = threadIdx.x; n < nn; n += kThreads) { cnt[n] = cnt[n] ? cnt[n] - st[n] : 0; }
+    __syncthreads();
+    // in-wave prefixes by ballot (layers below the last) and by a wave scan (last layer)
+    for (int n = threadIdx.x; n < nn; n += kThreads) {
+        const int c = cnt[n];
+        const int w = n >> 6;
+#pragma unroll
+        for (int l = 0; l < kLayers - 1; ++l) {
+            const uint64_t m = __ballot(c > l);
+            pre[l][n] = __popcll(m & lt_mask);
+            if (lane == 0) { tot[l][w] = __popcll(m); }
+        }
+        const int over = c > kLayers - 1 ? c - (kLayers - 1) : 0;
+        int incl = over;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int t = __shfl_up(incl, d, 64);
+            if (lane >= d) { incl += t; }
+        }
+        pre[kLayers - 1][n] = incl - over;
+        if (lane == 63) { tot[kLayers - 1][w] = incl; }
+    }
+    __syncthreads();
+    const int nw = (nn + 63) >> 6;
+    for (int n = threadIdx.x; n < nn; n += kThreads) {
+        const int w = n >> 6;
+#pragma unroll
+        for (int l = 0; l < kLayers; ++l) {
+            int carry = 0;
+            for (int v = 0; v < w; ++v) { carry += tot[l][v]; }
+            pre[l][n] += carry;
+        }
+    }
+    if (threadIdx.x == 0) {
+        int o = 0;
+        for (int l = 0; l < kLayers; ++l) {
+            loff[l] = o;
+            for (int v = 0; v < nw; ++v) { o += tot[l][v]; }
+        }
+    }
+    __syncthreads();
+    for (int j = s + threadIdx.x; j < e; j += kThreads) {
+        const uint32_t key = keys_in[j];
+        const int n = (int)(key & mask);
+        const int r = j - st[n];
+        const int off = r < kLayers - 1 ? loff[r] + pre[r][n] : loff[kLayers - 1] + pre[kLayers - 1][n] + (r - (kLayers - 1));
+        perm_out[s + off] = perm_in[j];
+        keys_out[s + off] = key >> node_bits;
+    }
 }
 // Work items per bin: ceil(count / chunk).
 __global__ __launch_bounds__(kThreads) void k_item_counts(const int32_t* bstart, const int32_t* bend, int nkeys,
@@ -538,12 +627,12 @@ hipError_t launch_binning(hipStream_t st, const DevParams& p, const Store& src, 
     const unsigned pblocks = (unsigned)((cap + kThreads - 1) / kThreads);
     hipLaunchKernelGGL(k_bin_keys, dim3(pblocks), dim3(kThreads), 0, st, p, src, count, ss.keys_tmp);
     const int passes = (ss.key_bits + kRadixBits - 1) / kRadixBits;
+    if (passes == 0) { return hipErrorInvalidValue; }
     const uint32_t* kin = ss.keys_tmp;
     const int32_t* vin = nullptr;
     for (int pass = 0; pass < passes; ++pass) {
-        const bool last = pass == passes - 1;
-        uint32_t* kout = last ? dst.key : (pass % 2 == 0 ? ss.keys_alt : ss.keys_tmp);
-        int32_t* vout = last ? ss.perm : (pass % 2 == 0 ? ss.vals_alt : ss.vals_tmp);
+        uint32_t* kout = pass % 2 == 0 ? ss.keys_alt : ss.keys_tmp;
+        int32_t* vout = pass % 2 == 0 ? ss.vals_alt : ss.vals_tmp;
         const int shift = pass * kRadixBits;
         hipLaunchKernelGGL(k_radix_upsweep, dim3(ss.ntiles), dim3(kThreads), 0, st, kin, count, shift, ss.hist,
                            ss.ntiles);
@@ -554,14 +643,16 @@ hipError_t launch_binning(hipStream_t st, const DevParams& p, const Store& src, 
         kin = kout;
         vin = vout;
     }
-    if (passes == 0) { return hipErrorInvalidValue; }
-    hipLaunchKernelGGL(k_gather, dim3(pblocks), dim3(kThreads), 0, st, src, dst, ss.perm, count);
+    // kin / vin: keys sorted by (bin, node) and the permutation; bin ranges, then the layered order
     hipError_t e = launch_fill_i32(st, ss.bin_start, (size_t)ss.num_keys, 0);
     if (e != hipSuccess) { return e; }
     e = launch_fill_i32(st, ss.bin_end, (size_t)ss.num_keys, 0);
     if (e != hipSuccess) { return e; }
-    hipLaunchKernelGGL(k_bin_bounds, dim3(pblocks), dim3(kThreads), 0, st, dst.key, count, ss.num_keys, ss.bin_start,
-                       ss.bin_end);
+    hipLaunchKernelGGL(k_bin_bounds, dim3(pblocks), dim3(kThreads), 0, st, kin, count, ss.num_keys, p.node_bits,
+                       ss.bin_start, ss.bin_end);
+    hipLaunchKernelGGL(k_layer_bins, dim3((unsigned)ss.num_keys), dim3(kThreads), 0, st, kin, vin, ss.bin_start,
+                       ss.bin_end, ss.num_keys, p.node_bits, dst.key, ss.perm);
+    hipLaunchKernelGGL(k_gather, dim3(pblocks), dim3(kThreads), 0, st, src, dst, ss.perm, count);
     const unsigned kblocks = (unsigned)((ss.num_keys + kThreads - 1) / kThreads);
     hipLaunchKernelGGL(k_item_counts, dim3(kblocks), dim3(kThreads), 0, st, ss.bin_start, ss.bin_end, ss.num_keys,
                        ss.chunk, ss.item_cnt);

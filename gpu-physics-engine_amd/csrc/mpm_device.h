@@ -43,6 +43,7 @@ struct DevParams {
     int num_scenes;
     int nbx, nby, bps;  // bins per scene along x, y and in total
     int bin_x, bin_y;   // cells per bin
+    int node_bits;      // log2(bin_x * bin_y): bits of a stencil base node within its bin
     float dx, inv_dx, sim_w, sim_h, dt;
     float gdt_x, gdt_y;  // gravity * dt, folded once like the reference's constexpr kGravity * dt (MPM.cu:154)
     float mass, vol;
@@ -339,13 +340,15 @@ __device__ __forceinline__ float2 node_velocity(const DevParams& p, float mx, fl
     return v;
 }
 
-// Bin key of a position (scene-major); base node clamped for memory safety.
-__device__ __forceinline__ uint32_t bin_key(const DevParams& p, float x, float y, uint32_t scene)
+// Bin key of a position (scene-major) and the stencil base node within the bin (row-major, x fastest);
+// base node clamped for memory safety.
+__device__ __forceinline__ uint32_t bin_key(const DevParams& p, float x, float y, uint32_t scene, uint32_t& node)
 {
     int bx = (int)floorf(fmaf(x, p.inv_dx, (float)p.halo) - 0.5f);
     int by = (int)floorf(fmaf(y, p.inv_dx, (float)p.halo) - 0.5f);
     bx = clampi(bx, 0, p.W - 3);
     by = clampi(by, 0, p.H - 3);
+    node = (uint32_t)((by % p.bin_y) * p.bin_x + bx % p.bin_x);
     return scene * (uint32_t)p.bps + (uint32_t)((by / p.bin_y) * p.nbx + bx / p.bin_x);
 }
 

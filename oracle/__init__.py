@@ -52,6 +52,8 @@ def lib():
         L.oracle_mlp_loss_grad.restype = ctypes.c_double
         L.oracle_adam.argtypes = [ctypes.c_int32, _d, _d, _d, _d, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                   ctypes.c_double, ctypes.c_int32]
+        L.oracle_bin_nodes_f32.argtypes = [vp, ctypes.c_int32, _f32, ctypes.c_int32, ctypes.c_int32, _u32]
+        L.oracle_layered_order.argtypes = [ctypes.c_int32, _u32, _u32, ctypes.c_int32, _i32]
         L.oracle_render.argtypes = [ctypes.c_int32, _f32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                     ctypes.c_float, ctypes.c_float, _u8]
         _lib = L
@@ -177,6 +179,27 @@ def stable_order(keys):
     keys = np.ascontiguousarray(keys, dtype=np.uint32)
     perm = np.zeros(keys.shape[0], dtype=np.int32)
     lib().oracle_stable_order(keys.shape[0], _p(keys, _u32), _p(perm, _i32))
+    return perm
+
+
+def bin_nodes(params, x, block_x, block_y):
+    """Stencil base node of each particle within its bin (row-major, x fastest)."""
+    x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 2)
+    nodes = np.zeros(x.shape[0], dtype=np.uint32)
+    lib().oracle_bin_nodes_f32(ctypes.byref(params), x.shape[0], _p(x, _f32), int(block_x), int(block_y),
+                               _p(nodes, _u32))
+    return nodes
+
+
+LAYERS = 16
+
+
+def layered_order(keys, nodes, layers=LAYERS):
+    """Binned order of the engine: bins ascending, inside a bin layers of one particle per stencil node."""
+    keys = np.ascontiguousarray(keys, dtype=np.uint32)
+    nodes = np.ascontiguousarray(nodes, dtype=np.uint32)
+    perm = np.zeros(keys.shape[0], dtype=np.int32)
+    lib().oracle_layered_order(keys.shape[0], _p(keys, _u32), _p(nodes, _u32), int(layers), _p(perm, _i32))
     return perm
 
 

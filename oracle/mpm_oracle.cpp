@@ -27,9 +27,9 @@
 // Besides the serial loop (deterministic), an OpenMP loop with per-thread private grids reduced in fixed
 // thread order is provided (deterministic for a given thread count); it is the timed CPU baseline.
 //
-// Binning restatement: FP32 bin keys computed exactly as the engine computes them (fmaf(x, inv_dx, halo),
-// the contracted form of gridCoord, MPM.cu:84) and std::stable_sort — the bit-exact order the engine's
-// counting sort must reproduce.
+// Binning restatement: FP32 bin keys and in-bin stencil nodes computed exactly as the engine computes them
+// (fmaf(x, inv_dx, halo), the contracted form of gridCoord, MPM.cu:84), std::stable_sort, and the layered
+// order inside bins (oracle_layered_order) — the bit-exact order the engine's binning must reproduce.
 //
 // MLP restatement: forward, backward and Adam of the 4-64-64-3 SiLU closure in FP64, for gradient checks.
 //
@@ -514,6 +514,46 @@ void oracle_stable_order(int32_t n, const uint32_t* keys, int32_t* perm)
     std::vector<int32_t> idx(n);
     std::iota(idx.begin(), idx.end(), 0);
     std::stable_sort(idx.begin(), idx.end(), [&](int32_t a, int32_t b) { return keys[a] < keys[b]; });
+    std::memcpy(perm, idx.data(), sizeof(int32_t) * (size_t)n);
+}
+
+// Stencil base node of each particle within its bin (row-major, x fastest), from the same FP32 base node
+// as oracle_bin_keys_f32.
+void oracle_bin_nodes_f32(const MpmParams* params, int32_t n, const float* x, int32_t block_x, int32_t block_y,
+                          uint32_t* nodes)
+{
+    const float h = (float)params->halo;
+    for (int i = 0; i < n; ++i) {
+        int bx = (int)std::floor(std::fmaf(x[2 * i], params->inv_dx, h) - 0.5f);
+        int by = (int)std::floor(std::fmaf(x[2 * i + 1], params->inv_dx, h) - 0.5f);
+        bx = std::min(std::max(bx, 0), params->grid_w - 3);
+        by = std::min(std::max(by, 0), params->grid_h - 3);
+        nodes[i] = (uint32_t)((by % block_y) * block_x + bx % block_x);
+    }
+}
+// Layered binned order: particles grouped by bin key (ascending); inside a bin, the particle of rank r
+// among those of its node (in input order) goes to layer min(r, layers - 1); layers in order, nodes in
+// order inside a layer, and the last layer in (node, rank) order.  perm[j] = input index of position j.
+void oracle_layered_order(int32_t n, const uint32_t* keys, const uint32_t* nodes, int32_t layers, int32_t* perm)
+{
+    std::vector<int32_t> idx(n);
+    std::iota(idx.begin(), idx.end(), 0);
+    std::stable_sort(idx.begin(), idx.end(), [&](int32_t a, int32_t b) {
+        return keys[a] != keys[b] ? keys[a] < keys[b] : nodes[a] < nodes[b];
+    });
+    std::vector<int32_t> rank(n);
+    for (int j = 0; j < n; ++j) {
+        const int32_t i = idx[j];
+        const bool same = j > 0 && keys[idx[j - 1]] == keys[i] && nodes[idx[j - 1]] == nodes[i];
+        rank[i] = same ? rank[idx[j - 1]] + 1 : 0;
+    }
+    std::stable_sort(idx.begin(), idx.end(), [&](int32_t a, int32_t b) {
+        if (keys[a] != keys[b]) { return keys[a] < keys[b]; }
+        const int32_t la = std::min(rank[a], layers - 1), lb = std::min(rank[b], layers - 1);
+        if (la != lb) { return la < lb; }
+        if (nodes[a] != nodes[b]) { return nodes[a] < nodes[b]; }
+        return rank[a] < rank[b];
+    });
     std::memcpy(perm, idx.data(), sizeof(int32_t) * (size_t)n);
 }
 

@@ -383,6 +383,37 @@ def test_bin_keys_and_stable_order(oracle_mod):
     scene = np.repeat(np.arange(5), 1000).astype(np.int32)
     k2 = oracle_mod.bin_keys(p, x, 16, 16, scene)
     assert np.array_equal(k2, keys + scene.astype(np.uint32) * nbx * ((p.grid_h + 15) // 16))
+    nodes = oracle_mod.bin_nodes(p, x, 16, 8)
+    assert np.array_equal(nodes, (by % 8) * 16 + bx % 16)
+
+
+def test_layered_order(oracle_mod):
+    """Hand-worked cases of the layered in-bin order (rank r of a particle among those of its node, layers
+    min(r, L-1) in order, nodes in order inside a layer, the last layer in (node, rank) order), then its
+    properties on random input against a brute-force restatement."""
+    k = np.zeros(6, np.uint32)
+    assert oracle_mod.layered_order(k, np.array([5, 3, 5, 5, 3, 1], np.uint32)).tolist() == [5, 1, 0, 4, 2, 3]
+    n2 = np.array([7, 7, 7, 2, 2, 2], np.uint32)
+    assert oracle_mod.layered_order(k, n2).tolist() == [3, 0, 4, 1, 5, 2]
+    assert oracle_mod.layered_order(k, n2, layers=2).tolist() == [3, 0, 4, 5, 1, 2]
+    rng = np.random.default_rng(12)
+    keys = rng.integers(0, 7, 3000).astype(np.uint32)
+    nodes = np.minimum(rng.geometric(0.08, 3000) - 1, 127).astype(np.uint32)  # some nodes above 16 particles
+    perm = oracle_mod.layered_order(keys, nodes)
+    assert np.array_equal(np.sort(perm), np.arange(3000))
+    assert np.all(np.diff(keys[perm].astype(np.int64)) >= 0)  # bins contiguous and ascending
+    rank = np.zeros(3000, np.int64)
+    seen = {}
+    for i in range(3000):  # rank in input order
+        rank[i] = seen.get((keys[i], nodes[i]), 0)
+        seen[(keys[i], nodes[i])] = rank[i] + 1
+    layer = np.minimum(rank, 15)
+    brute = sorted(range(3000), key=lambda i: (keys[i], layer[i], nodes[i], rank[i]))
+    assert perm.tolist() == brute
+    for b in range(7):  # every layer below the last holds distinct nodes
+        for l in range(15):
+            sel = (keys == b) & (layer == l)
+            assert len(np.unique(nodes[sel])) == sel.sum()
 
 
 # ---------------------------------------------------------------------------------------------- MLP
