@@ -874,19 +874,21 @@ struct CeilLog2<1> {
     static constexpr int value = 0;
 };
 
-// Scatter of one particle into its wave's LDS accumulator tile, without atomics.  Lanes of the wave are
-// ranked among the lanes whose stencil starts at the same node (an exact multisplit over the node index
-// with one ballot per bit); the lanes of one rank start at distinct nodes, so for each of the 9 stencil
-// offsets their targets are distinct and a plain read-add-write is race free.  Ranks are processed in
-// ascending order and the 9 offsets in a fixed order (the compiler barrier keeps every read after the
-// previous write of the same wave; LDS executes a wave's instructions in order), so the summation order
-// is fixed by the particle order.  A particle that drifted out of the tile scatters to HBM atomically.
-template <int BX, int BY>
+// Scatter of one particle into its wave's LDS accumulator tile, without atomics, in rounds in which the
+// updating lanes start their stencils at distinct nodes, so for each of the 9 offsets their targets are
+// distinct and a plain read-add-write is race free (the compiler barriers keep every read after the
+// previous write of the same wave; LDS executes a wave's instructions in order).  Two ways to form rounds:
+//  - ranked (default): lanes are ranked among those starting at the same node by an exact multisplit over
+//    the node index (one ballot per bit); round r takes rank r, so the summation order is the particle order;
+//  - claimed (kClaim, for the VALU-bound learned kernels): each pending lane writes its lane index into the
+//    unused fourth word of its start node and the lane that reads its own index back updates; the others
+//    retry.  No ballots per bit, two more LDS instructions per round.
+// A particle that drifted out of the tile scatters to HBM atomically.
+template <int BX, int BY, bool kClaim = false>
 __device__ __forceinline__ void scatter(const DevParams& p, const Grid& g, size_t gb, int ox, int oy, float4* wtile,
                                         float px, float py, float vx, float vy, const Mat2& Q, bool active = true)
 {
     using T = TileShape<BX, BY>;
-    constexpr int kBits = CeilLog2<T::N>::value;
     int bx, by;
     float fx, fy, wx[3], wy[3];
     bspline(px, p.inv_dx, (float)p.halo, bx, fx, wx);
@@ -932,34 +934,63 @@ __device__ __forceinline__ void scatter(const DevParams& p, const Grid& g, size_
     }
     const int base = ly * T::TX + lx;
     const int lane = threadIdx.x & 63;
-    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    uint64_t peers = __ballot(inside);
+    if constexpr (kClaim) {
+        // one claimant per start node reads its own index back (the barrier keeps the read after the write)
+        bool pending = inside;
+        while (__ballot(pending) != 0ull) {
+            float* claim = &wtile[base].w;
+            if (pending) { *claim = __int_as_float(lane); }
+            __asm__ __volatile__("" ::: "memory");
+            const bool won = pending && __float_as_int(*claim) == lane;
+            __asm__ __volatile__("" ::: "memory");
+            if (won) {
 #pragma unroll
-    for (int k = 0; k < kBits; ++k) {
-        const bool bit = (base >> k) & 1;
-        const uint64_t m = __ballot(inside && bit);
-        peers &= bit ? m : ~m;
-    }
-    const int rank = __popcll(peers & lt_mask);
-    // ranks r .. r + kTilesPerWave - 1 update in the same round, each in its own tile of the wave
-    for (int r = 0;; r += kTilesPerWave) {
-        if (inside && rank >= r && rank < r + kTilesPerWave) {
-            float4* tile = wtile + (rank - r) * T::N;
+                for (int a = 0; a < 3; ++a) {
 #pragma unroll
-            for (int a = 0; a < 3; ++a) {
-#pragma unroll
-                for (int b = 0; b < 3; ++b) {
-                    float4* cell = &tile[base + b * T::TX + a];
-                    float4 v = *cell;
-                    v.x += cx[a * 3 + b];
-                    v.y += cy[a * 3 + b];
-                    v.z += cm[a * 3 + b];
-                    *cell = v;
-                    __asm__ __volatile__("" ::: "memory");
+                    for (int b = 0; b < 3; ++b) {
+                        float4* cell = &wtile[base + b * T::TX + a];
+                        float4 v = *cell;
+                        v.x += cx[a * 3 + b];
+                        v.y += cy[a * 3 + b];
+                        v.z += cm[a * 3 + b];
+                        *cell = v;
+                        __asm__ __volatile__("" ::: "memory");
+                    }
                 }
+                pending = false;
             }
         }
-        if (__ballot(inside && rank >= r + kTilesPerWave) == 0ull) { break; }
+    } else {
+        constexpr int kBits = CeilLog2<T::N>::value;
+        const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+        uint64_t peers = __ballot(inside);
+#pragma unroll
+        for (int k = 0; k < kBits; ++k) {
+            const bool bit = (base >> k) & 1;
+            const uint64_t m = __ballot(inside && bit);
+            peers &= bit ? m : ~m;
+        }
+        const int rank = __popcll(peers & lt_mask);
+        // ranks r .. r + kTilesPerWave - 1 update in the same round, each in its own tile of the wave
+        for (int r = 0;; r += kTilesPerWave) {
+            if (inside && rank >= r && rank < r + kTilesPerWave) {
+                float4* tile = wtile + (rank - r) * T::N;
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+#pragma unroll
+                    for (int b = 0; b < 3; ++b) {
+                        float4* cell = &tile[base + b * T::TX + a];
+                        float4 v = *cell;
+                        v.x += cx[a * 3 + b];
+                        v.y += cy[a * 3 + b];
+                        v.z += cm[a * 3 + b];
+                        *cell = v;
+                        __asm__ __volatile__("" ::: "memory");
+                    }
+                }
+            }
+            if (__ballot(inside && rank >= r + kTilesPerWave) == 0ull) { break; }
+        }
     }
 }
 
@@ -1117,7 +1148,7 @@ __global__ __launch_bounds__(kThreads) void k_p2g_tiled_learned(DevParams p, Sto
         const uint32_t o = (uint32_t)(valid ? j : item.end - 1) * 4u;
         const int mat = ldo(s.mat, o >> 2);
         const Mat2 Q = affine_q_learned(p, &cl, mat, ld_C(s, o), ld_F(s, o), ldo(s.J, o));
-        scatter<BX, BY>(p, g, gb, ox, oy, wtile, ldo(s.x, o), ldo(s.y, o), ldo(s.vx, o), ldo(s.vy, o), Q, valid);
+        scatter<BX, BY, true>(p, g, gb, ox, oy, wtile, ldo(s.x, o), ldo(s.y, o), ldo(s.vx, o), ldo(s.vy, o), Q, valid);
     }
     __syncthreads();
     flush_acc_tiles<BX, BY>(p, g, gb, ox, oy, acc);
@@ -1165,7 +1196,7 @@ __global__ __launch_bounds__(kThreads) void k_g2p2g_learned(DevParams p, Store s
             sto(s.J, o, J);
         }
         const Mat2 Q = affine_q_learned(p, &cl, mat, C, F, J);
-        scatter<BX, BY>(p, g, gb, ox, oy, wtile, px, py, vx, vy, Q, valid);
+        scatter<BX, BY, true>(p, g, gb, ox, oy, wtile, px, py, vx, vy, Q, valid);
     }
     __syncthreads();
     flush_acc_tiles<BX, BY>(p, g, gb, ox, oy, acc);
