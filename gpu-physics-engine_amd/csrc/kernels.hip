@@ -971,15 +971,13 @@ __device__ __forceinline__ void scatter(const DevParams& p, const Grid& g, size_
             peers &= bit ? m : ~m;
         }
         const int rank = __popcll(peers & lt_mask);
-        // ranks r .. r + kTilesPerWave - 1 update in the same round, each in its own tile of the wave
-        for (int r = 0;; r += kTilesPerWave) {
-            if (inside && rank >= r && rank < r + kTilesPerWave) {
-                float4* tile = wtile + (rank - r) * T::N;
+        for (int r = 0;; ++r) {
+            if (inside && rank == r) {
 #pragma unroll
                 for (int a = 0; a < 3; ++a) {
 #pragma unroll
                     for (int b = 0; b < 3; ++b) {
-                        float4* cell = &tile[base + b * T::TX + a];
+                        float4* cell = &wtile[base + b * T::TX + a];
                         float4 v = *cell;
                         v.x += cx[a * 3 + b];
                         v.y += cy[a * 3 + b];
@@ -989,7 +987,7 @@ __device__ __forceinline__ void scatter(const DevParams& p, const Grid& g, size_
                     }
                 }
             }
-            if (__ballot(inside && rank >= r + kTilesPerWave) == 0ull) { break; }
+            if (__ballot(inside && rank > r) == 0ull) { break; }
         }
     }
 }
@@ -1012,7 +1010,7 @@ __global__ __launch_bounds__(kThreads) void k_p2g_tiled(DevParams p, Store s, Gr
     const size_t gb = (size_t)scene * (size_t)p.G;
     zero_acc_tiles<BX, BY>(acc);
     __syncthreads();
-    float4* wtile = acc + (threadIdx.x >> 6) * kTilesPerWave * T::N;
+    float4* wtile = acc + (threadIdx.x >> 6) * T::N;
     for (int j = item.begin + threadIdx.x; j < item.end; j += kThreads) {
         const uint32_t o = (uint32_t)j * 4u;
         const Mat2 Q = affine_q(p, ldo(s.mat, (uint32_t)j), ld_C(s, o), ld_F(s, o), ldo(s.J, o));
@@ -1080,7 +1078,7 @@ __global__ __launch_bounds__(kThreads) void k_g2p2g(DevParams p, Store s, Grid g
     load_vel_tile<BX, BY>(p, g, gb, ox, oy, sv);
     zero_acc_tiles<BX, BY>(acc);
     __syncthreads();
-    float4* wtile = acc + (threadIdx.x >> 6) * kTilesPerWave * T::N;
+    float4* wtile = acc + (threadIdx.x >> 6) * T::N;
     for (int j = item.begin + threadIdx.x; j < item.end; j += kThreads) {
         const uint32_t o = (uint32_t)j * 4u;
         float px = ldo(s.x, o), py = ldo(s.y, o), vx, vy;
@@ -1141,7 +1139,7 @@ __global__ __launch_bounds__(kThreads) void k_p2g_tiled_learned(DevParams p, Sto
     zero_acc_tiles<BX, BY>(acc);
     closure_stage(p.cl, &cl);
     __syncthreads();
-    float4* wtile = acc + (threadIdx.x >> 6) * kTilesPerWave * T::N;
+    float4* wtile = acc + (threadIdx.x >> 6) * T::N;
     for (int base = item.begin; base < item.end; base += kThreads) {
         const int j = base + threadIdx.x;
         const bool valid = j < item.end;
@@ -1178,7 +1176,7 @@ __global__ __launch_bounds__(kThreads) void k_g2p2g_learned(DevParams p, Store s
     zero_acc_tiles<BX, BY>(acc);
     closure_stage(p.cl, &cl);
     __syncthreads();
-    float4* wtile = acc + (threadIdx.x >> 6) * kTilesPerWave * T::N;
+    float4* wtile = acc + (threadIdx.x >> 6) * T::N;
     for (int base = item.begin; base < item.end; base += kThreads) {
         const int j = base + threadIdx.x;
         const bool valid = j < item.end;

@@ -19,13 +19,7 @@ constexpr int kRadixBins = 1 << kRadixBits;
 constexpr int kRadixItems = 16;      // keys per thread per sort tile
 constexpr int kRadixTile = kThreads * kRadixItems;
 constexpr int kMargin = 2;           // nodes of drift a binned particle may take before leaving its tile
-#ifndef MPM_TILES_PER_WAVE
-#define MPM_TILES_PER_WAVE 1
-#endif
-// LDS accumulation tiles per wave: lanes whose stencils start at the same node update different tiles in
-// one round, so a wave needs ceil(multiplicity / kTilesPerWave) rounds.
-constexpr int kTilesPerWave = MPM_TILES_PER_WAVE;
-constexpr int kAccTiles = kTilesPerWave * (kThreads / 64);  // tiles per work group
+constexpr int kAccTiles = kThreads / 64;  // LDS accumulation tiles per work group (one per wave)
 constexpr int kMaxScenes = 4096;
 
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
