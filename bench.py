@@ -140,8 +140,8 @@ def main():
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported (multi-GPU sharding "
-              f"is a design note, not built)", file=sys.stderr)
+        print(f"bench.py: the engine runs on one GPU; --gpus {args.gpus} is not supported (sharding across GPUs "
+              f"is designed, not implemented)", file=sys.stderr)
         sys.exit(2)
 
     preset, make_scene, description = WORKLOADS[args.workload]

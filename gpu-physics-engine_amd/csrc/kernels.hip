@@ -9,8 +9,9 @@
 //   (P2G of s+1 fused with G2P of s, no reference counterpart) -> k_g2p2g
 //   MPMOutputKernel   :302-322 -> k_output
 //   MPMEmitterKernel  :380-438 -> k_emitter (own Philox4x32-10 + Box-Muller, wave64 ballot)
-// plus the stable counting/radix binning that the reference MPM does not have (the reference's only
-// sort, PBF.cu:738-834, ranks with atomicAdd and is order-nondeterministic; this one is stable).
+// plus the binning that the reference MPM does not have: a stable radix sort by (bin, stencil base node)
+// and a deterministic layered order inside each bin (the reference's only sort, PBF.cu:738-834, ranks with
+// atomicAdd and is order-nondeterministic).
 #include "kernels.h"
 #include "closure.h"
 
@@ -277,7 +278,7 @@ hipError_t launch_g2p_reference(hipStream_t st, const DevParams& p, const Store&
     return hipGetLastError();
 }
 
-// ===================================================================================== binning (stable)
+// ===================================================================================== binning (stable, layered)
 
 // Sort key of every particle from its position: bin key (the scene comes from the previous key) above the
 // stencil base node within the bin.

@@ -26,9 +26,10 @@ extern "C" {
 #define MPM_KT_OUTPUT 5  /* k_output                                                                       */
 #define MPM_KT_COUNT 6
 
-/* Bin the current state (stable counting sort by bin key, exactly as a tick does first) and copy the
- * binned order out: keys[j] = bin key of the j-th particle, ids[j] = its caller index.  Arrays sized
- * for the current count; *n receives it.  Not available in MPM_MODE_REFERENCE (returns MPM_ERR_STATE). */
+/* Bin the current state exactly as a tick does first (stable radix sort by bin key and stencil base node
+ * within the bin, then layers of one particle per node inside each bin) and copy the binned order out:
+ * keys[j] = bin key of the j-th particle, ids[j] = its caller index.  Arrays sized for the current count;
+ * *n receives it.  Not available in MPM_MODE_REFERENCE (returns MPM_ERR_STATE). */
 int32_t mpm_debug_bin(MpmEngine* h, uint32_t* keys, int32_t* ids, int32_t* n);
 
 /* Advance `substeps` substeps with the handle's mode, launching eagerly (not through the tick graph) and
