@@ -138,6 +138,31 @@ def test_mixed1024_scaled_against_oracle(oracle_mod):
     assert rel(out["grid"][0][0], grid[0]) < 1e-4
 
 
+@pytest.mark.parametrize("mode", MODES)
+def test_reference_preset_against_oracle(oracle_mod, mode):
+    """The reference configuration itself (constants.h: dx = 2, 196 x 100 nodes, Tait k = 8.2e4,
+    gamma = 3.2 through powf, gravity +700 with y down, dt = 1.6e-3, 20 substeps per tick): a jittered
+    fluid block moving at the hose-pipe speed (12 units per tick), 5 ticks, against the FP64 oracle."""
+    p = abi.preset(abi.PRESET_REFERENCE)
+    p.mode = mode
+    rng = np.random.default_rng(21)
+    gx, gy = np.meshgrid(np.arange(48), np.arange(40))
+    x = np.stack([60.0 + 1.26 * gx.ravel(), 90.0 + 1.26 * gy.ravel()], 1)
+    x = (x + rng.normal(0, 0.252, x.shape)).astype(np.float32)
+    v = np.tile(np.float32([12.0 / 0.032, 0.0]), (x.shape[0], 1))
+    mat = np.zeros(x.shape[0], np.uint8)
+    p.capacity = x.shape[0]
+    out = run_gpu(p, x, v, mat, ticks=5)
+    s, _ = run_oracle(oracle_mod, p, x, v, mat, 5 * p.substeps)
+    print(f"[reference preset {MODE_NAMES[mode]}] rel x {rel(out['x'], s.x):.2e} v {rel(out['v'], s.v):.2e} "
+          f"J {rel(out['J'], s.J):.2e}", flush=True)
+    # FP32 positions near 100 units carry ~8e-8 relative rounding per operation; 100 substeps of a stiff
+    # Tait fluid (powf with gamma = 3.2) accumulate about 1e-6
+    assert rel(out["x"], s.x) < 1e-5
+    assert rel(out["v"], s.v) < 1e-4
+    assert rel(out["J"], s.J) < 1e-5
+
+
 @pytest.mark.parametrize("shape", [(8, 8), (16, 8), (16, 16), (32, 16)])
 def test_binning_bit_exact(oracle_mod, shape):
     """The engine's binning equals the layered order of the oracle (stable sort by FP32 bin key and stencil
