@@ -643,14 +643,14 @@ int32_t mpm_params_preset(int32_t preset, MpmParams* out)
         p.substeps = 200;
         p.num_scenes = 64;
         p.capacity = 64 * 524288;
-        p.sort_interval = 25;
+        p.sort_interval = 100;
         break;
     case MPM_PRESET_LEARNED1024:
         unit_square(1024, 2e-5f, 16.f);
         lame(1e4f, 0.2f);
         p.substeps = 100;
         p.capacity = 4194304;
-        p.sort_interval = 25;
+        p.sort_interval = 100;
         p.closure = MPM_CLOSURE_LEARNED;
         break;
     case MPM_PRESET_LEARNED16X2048:
@@ -659,7 +659,7 @@ int32_t mpm_params_preset(int32_t preset, MpmParams* out)
         p.substeps = 100;
         p.num_scenes = 16;
         p.capacity = 16 * 16777216 / 2;
-        p.sort_interval = 25;
+        p.sort_interval = 100;
         p.closure = MPM_CLOSURE_LEARNED;
         break;
     default:
