@@ -44,6 +44,16 @@ struct SortScratch {
 constexpr int kNumShapes = 4;
 constexpr int kShapeX[kNumShapes] = {8, 16, 16, 32};
 constexpr int kShapeY[kNumShapes] = {8, 8, 16, 16};
+// The binning's checkerboard slots need an even bin_x and bin_x * bin_y a power of two of at least 64.
+constexpr bool shapes_fit_binning()
+{
+    for (int k = 0; k < kNumShapes; ++k) {
+        const int nn = kShapeX[k] * kShapeY[k];
+        if (kShapeX[k] % 2 != 0 || nn < 64 || (nn & (nn - 1)) != 0) { return false; }
+    }
+    return true;
+}
+static_assert(shapes_fit_binning(), "tile shape unsuited to the binning");
 int shape_index(int bx, int by);
 
 hipError_t launch_fill_f32(hipStream_t st, float* p, size_t n, float v);

@@ -508,14 +508,30 @@ __global__ __launch_bounds__(kThreads) void k_bin_bounds(const uint32_t* keys, c
 
 // Layered order within each bin.  Input: particles sorted stably by (bin, stencil base node).  Particle j
 // of node n has rank r = j - (first index of n); it goes to layer min(r, kLayers - 1).  Layers follow one
-// another, and inside a layer below the last the particles follow in node order, so the 64 lanes of a wave
-// mostly start their stencils at distinct nodes and the ranked scatter needs fewer rounds.  The last layer
-// takes the remaining particles in (node, rank) order.  One work group per bin.
+// another, and inside a layer below the last each node holds one particle, so the 64 lanes of a wave
+// mostly start their stencils at distinct nodes and the ranked scatter needs fewer rounds.  Inside a layer
+// the nodes follow in checkerboard order: first those with lx + ly even, then those with lx + ly odd, each
+// group in node order.  The lanes of a wave then start two cells (or one diagonal step) apart, so particles
+// that drift by a cell during the substeps between two sorts still mostly start at distinct nodes.  The
+// last layer takes the remaining particles in (checkerboard slot, rank) order.  One work group per bin.
 constexpr int kLayers = 16;
 constexpr int kMaxBinNodes = 512;  // largest bin (32 x 16 cells)
+
+// Node (ly * bin_x + lx) at checkerboard slot q of a bin of nn = bin_x * bin_y nodes, bin_x even: slots
+// [0, nn/2) are the nodes with lx + ly even, slots [nn/2, nn) those with lx + ly odd, each in node order.
+__device__ __forceinline__ int checker_node(int q, int nn, int bin_x)
+{
+    const int c = q >= (nn >> 1) ? 1 : 0;
+    const int r = q - c * (nn >> 1);
+    const int hx = bin_x >> 1;
+    const int ly = r / hx;
+    const int lx = 2 * (r - ly * hx) + ((ly + c) & 1);
+    return ly * bin_x + lx;
+}
 __global__ __launch_bounds__(kThreads) void k_layer_bins(const uint32_t* keys_in, const int32_t* perm_in,
                                                          const int32_t* bstart, const int32_t* bend, int nkeys,
-                                                         int node_bits, uint32_t* keys_out, int32_t* perm_out)
+                                                         int node_bits, int bin_x, uint32_t* keys_out,
+                                                         int32_t* perm_out)
 {
     __shared__ int st[kMaxBinNodes];                 // first index of each node
     __shared__ int cnt[kMaxBinNodes];                // particles of each node
@@ -543,10 +559,11 @@ __global__ __launch_bounds__(kThreads) void k_layer_bins(const uint32_t* keys_in
     __syncthreads();
     for (int n = threadIdx.x; n < nn; n += kThreads) { cnt[n] = cnt[n] ? cnt[n] - st[n] : 0; }
     __syncthreads();
-    // in-wave prefixes by ballot (layers below the last) and by a wave scan (last layer)
-    for (int n = threadIdx.x; n < nn; n += kThreads) {
+    // in-wave prefixes over checkerboard slots by ballot (layers below the last) and by a wave scan (last)
+    for (int q = threadIdx.x; q < nn; q += kThreads) {
+        const int n = checker_node(q, nn, bin_x);
         const int c = cnt[n];
-        const int w = n >> 6;
+        const int w = q >> 6;
 #pragma unroll
         for (int l = 0; l < kLayers - 1; ++l) {
             const uint64_t m = __ballot(c > l);
@@ -565,8 +582,9 @@ __global__ __launch_bounds__(kThreads) void k_layer_bins(const uint32_t* keys_in
     }
     __syncthreads();
     const int nw = (nn + 63) >> 6;
-    for (int n = threadIdx.x; n < nn; n += kThreads) {
-        const int w = n >> 6;
+    for (int q = threadIdx.x; q < nn; q += kThreads) {
+        const int n = checker_node(q, nn, bin_x);
+        const int w = q >> 6;
 #pragma unroll
         for (int l = 0; l < kLayers; ++l) {
             int carry = 0;
@@ -652,7 +670,7 @@ hipError_t launch_binning(hipStream_t st, const DevParams& p, const Store& src, 
     hipLaunchKernelGGL(k_bin_bounds, dim3(pblocks), dim3(kThreads), 0, st, kin, count, ss.num_keys, p.node_bits,
                        ss.bin_start, ss.bin_end);
     hipLaunchKernelGGL(k_layer_bins, dim3((unsigned)ss.num_keys), dim3(kThreads), 0, st, kin, vin, ss.bin_start,
-                       ss.bin_end, ss.num_keys, p.node_bits, dst.key, ss.perm);
+                       ss.bin_end, ss.num_keys, p.node_bits, p.bin_x, dst.key, ss.perm);
     hipLaunchKernelGGL(k_gather, dim3(pblocks), dim3(kThreads), 0, st, src, dst, ss.perm, count);
     const unsigned kblocks = (unsigned)((ss.num_keys + kThreads - 1) / kThreads);
     hipLaunchKernelGGL(k_item_counts, dim3(kblocks), dim3(kThreads), 0, st, ss.bin_start, ss.bin_end, ss.num_keys,

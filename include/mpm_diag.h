@@ -27,7 +27,8 @@ extern "C" {
 #define MPM_KT_COUNT 6
 
 /* Bin the current state exactly as a tick does first (stable radix sort by bin key and stencil base node
- * within the bin, then layers of one particle per node inside each bin) and copy the binned order out:
+ * within the bin, then layers of one particle per node inside each bin, the nodes of a layer in checkerboard
+ * order: lx + ly even first, then odd, each in node order) and copy the binned order out:
  * keys[j] = bin key of the j-th particle, ids[j] = its caller index.  Arrays sized for the current count;
  * *n receives it.  Not available in MPM_MODE_REFERENCE (returns MPM_ERR_STATE). */
 int32_t mpm_debug_bin(MpmEngine* h, uint32_t* keys, int32_t* ids, int32_t* n);

@@ -54,6 +54,7 @@ def lib():
                                   ctypes.c_double, ctypes.c_int32]
         L.oracle_bin_nodes_f32.argtypes = [vp, ctypes.c_int32, _f32, ctypes.c_int32, ctypes.c_int32, _u32]
         L.oracle_layered_order.argtypes = [ctypes.c_int32, _u32, _u32, ctypes.c_int32, _i32]
+        L.oracle_checker_slots.argtypes = [ctypes.c_int32, _u32, ctypes.c_int32, ctypes.c_int32, _u32]
         L.oracle_render.argtypes = [ctypes.c_int32, _f32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                     ctypes.c_float, ctypes.c_float, _u8]
         _lib = L
@@ -189,6 +190,20 @@ def bin_nodes(params, x, block_x, block_y):
     lib().oracle_bin_nodes_f32(ctypes.byref(params), x.shape[0], _p(x, _f32), int(block_x), int(block_y),
                                _p(nodes, _u32))
     return nodes
+
+
+def checker_slots(nodes, block_x, block_y):
+    """Checkerboard slot of each in-bin node: nodes with lx + ly even first, then odd, each in node order."""
+    nodes = np.ascontiguousarray(nodes, dtype=np.uint32)
+    slots = np.zeros(nodes.shape[0], dtype=np.uint32)
+    lib().oracle_checker_slots(nodes.shape[0], _p(nodes, _u32), int(block_x), int(block_y), _p(slots, _u32))
+    return slots
+
+
+def bin_slots(params, x, block_x, block_y):
+    """Checkerboard slot of each particle's stencil base node within its bin: the node label of the engine's
+    layered order."""
+    return checker_slots(bin_nodes(params, x, block_x, block_y), block_x, block_y)
 
 
 LAYERS = 16

@@ -531,9 +531,22 @@ void oracle_bin_nodes_f32(const MpmParams* params, int32_t n, const float* x, in
         nodes[i] = (uint32_t)((by % block_y) * block_x + bx % block_x);
     }
 }
+// Checkerboard slot of each in-bin node (ly * block_x + lx, block_x even): the nodes with lx + ly even take
+// slots 0 .. nb/2 - 1 in node order, those with lx + ly odd slots nb/2 .. nb - 1, nb = block_x * block_y.
+// Counted directly: slot = parity * nb/2 + (nodes of the same parity before it) = parity * nb/2 +
+// ly * block_x/2 + lx / 2, since each row holds block_x/2 nodes of each parity.
+void oracle_checker_slots(int32_t n, const uint32_t* nodes, int32_t block_x, int32_t block_y, uint32_t* slots)
+{
+    const uint32_t half = (uint32_t)(block_x * block_y / 2);
+    for (int i = 0; i < n; ++i) {
+        const uint32_t lx = nodes[i] % (uint32_t)block_x, ly = nodes[i] / (uint32_t)block_x;
+        slots[i] = ((lx + ly) & 1u) * half + ly * (uint32_t)(block_x / 2) + lx / 2;
+    }
+}
 // Layered binned order: particles grouped by bin key (ascending); inside a bin, the particle of rank r
 // among those of its node (in input order) goes to layer min(r, layers - 1); layers in order, nodes in
-// order inside a layer, and the last layer in (node, rank) order.  perm[j] = input index of position j.
+// ascending label order inside a layer, and the last layer in (node, rank) order.  The engine labels
+// nodes by their checkerboard slots (oracle_checker_slots).  perm[j] = input index of position j.
 void oracle_layered_order(int32_t n, const uint32_t* keys, const uint32_t* nodes, int32_t layers, int32_t* perm)
 {
     std::vector<int32_t> idx(n);

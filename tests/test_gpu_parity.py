@@ -166,7 +166,8 @@ def test_reference_preset_against_oracle(oracle_mod, mode):
 @pytest.mark.parametrize("shape", [(8, 8), (16, 8), (16, 16), (32, 16)])
 def test_binning_bit_exact(oracle_mod, shape):
     """The engine's binning equals the layered order of the oracle (stable sort by FP32 bin key and stencil
-    node, then layers of one particle per node) bit for bit, including a shuffled input and several scenes."""
+    node, then layers of one particle per node, nodes in checkerboard order) bit for bit, including a
+    shuffled input and several scenes."""
     p = abi.preset(abi.PRESET_BATCH64X256)
     p.num_scenes = 4
     p.block_x, p.block_y = shape
@@ -182,7 +183,7 @@ def test_binning_bit_exact(oracle_mod, shape):
         e.set_particles(x, material=np.ones(n, np.uint8), scene_counts=counts)
         keys, ids = e.debug_bin()
     ref_keys = oracle_mod.bin_keys(p, x, shape[0], shape[1], scene_of)
-    order = oracle_mod.layered_order(ref_keys, oracle_mod.bin_nodes(p, x, shape[0], shape[1]))
+    order = oracle_mod.layered_order(ref_keys, oracle_mod.bin_slots(p, x, shape[0], shape[1]))
     assert np.array_equal(ids, order)
     assert np.array_equal(keys, ref_keys[order])
 
@@ -195,7 +196,7 @@ def test_binning_large_random_order(oracle_mod):
         e.set_scene(sc)
         keys, ids = e.debug_bin()
     ref_keys = oracle_mod.bin_keys(p, sc.x, p.block_x, p.block_y)
-    order = oracle_mod.layered_order(ref_keys, oracle_mod.bin_nodes(p, sc.x, p.block_x, p.block_y))
+    order = oracle_mod.layered_order(ref_keys, oracle_mod.bin_slots(p, sc.x, p.block_x, p.block_y))
     assert np.array_equal(ids, order)
     assert np.array_equal(keys, ref_keys[order])
 

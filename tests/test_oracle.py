@@ -387,6 +387,25 @@ def test_bin_keys_and_stable_order(oracle_mod):
     assert np.array_equal(nodes, (by % 8) * 16 + bx % 16)
 
 
+def test_checker_slots(oracle_mod):
+    """Checkerboard slots: a hand-worked 4 x 2 bin, then on every compiled bin shape a bijection whose first
+    half holds the nodes with lx + ly even, no two of them 4-neighbours, each half in node order."""
+    # 4 x 2 bin: even nodes 0, 2, 5, 7 -> slots 0..3; odd nodes 1, 3, 4, 6 -> slots 4..7
+    assert oracle_mod.checker_slots(np.arange(8), 4, 2).tolist() == [0, 4, 1, 5, 6, 2, 7, 3]
+    for bx, by in [(8, 8), (16, 8), (16, 16), (32, 16)]:
+        nb = bx * by
+        nodes = np.arange(nb, dtype=np.uint32)
+        slots = oracle_mod.checker_slots(nodes, bx, by).astype(np.int64)
+        assert np.array_equal(np.sort(slots), np.arange(nb))
+        lx, ly = nodes % bx, nodes // bx
+        even = (lx + ly) % 2 == 0
+        assert np.all(slots[even] < nb // 2) and np.all(slots[~even] >= nb // 2)
+        assert np.all(np.diff(slots[even]) > 0) and np.all(np.diff(slots[~even]) > 0)
+        first = np.zeros((by, bx), bool)
+        first[ly[even], lx[even]] = True
+        assert not np.any(first[:, 1:] & first[:, :-1]) and not np.any(first[1:, :] & first[:-1, :])
+
+
 def test_layered_order(oracle_mod):
     """Hand-worked cases of the layered in-bin order (rank r of a particle among those of its node, layers
     min(r, L-1) in order, nodes in order inside a layer, the last layer in (node, rank) order), then its
