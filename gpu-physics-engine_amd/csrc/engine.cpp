@@ -166,6 +166,9 @@ std::string validate(MpmParams& p)
     }
     if (!(p.mass > 0.f) || !(p.vol > 0.f)) { return "mass and vol must be > 0"; }
     if (!(p.j_min > 0.f) || !(p.j_max >= p.j_min)) { return "J clamp range invalid"; }
+    if (!(p.theta_c >= 0.f && p.theta_c < 1.f) || !(p.theta_s >= 0.f)) {
+        return "snow thresholds: need 0 <= theta_c < 1 and theta_s >= 0";
+    }
     if (p.substeps < 1 || p.substeps > 100000) { return "substeps out of range"; }
     if (p.capacity < 1 || p.capacity > 1000000000) { return "capacity out of range"; }
     if (p.num_scenes < 1 || p.num_scenes > kMaxScenes) { return "num_scenes out of range"; }

@@ -127,13 +127,15 @@ __device__ __forceinline__ Mat2 mul2(const Mat2& A, const Mat2& B)
 
 // Rotation factor of the polar decomposition F = R S (the rotation maximising tr(R^T F)):
 // angle atan2(F10 - F01, F00 + F11), R = I when both vanish.  Returns cos, sin.
+// The FP32 form runs only for 0.1 <= det F <= 10, where r2 = |F|^2 + 2 det F >= 4 det F >= 0.4: the
+// hardware reciprocal square root (v_rsq_f32, 1 ulp) needs no denormal scaling there.
 __device__ __forceinline__ void polar_cs_f32(const Mat2& F, float& c, float& s)
 {
     const float x = F.a00 + F.a11;
     const float y = F.a10 - F.a01;
     const float r2 = x * x + y * y;
     if (r2 > 0.f) {
-        const float inv = rsqrtf(r2);
+        const float inv = __builtin_amdgcn_rsqf(r2);
         c = x * inv;
         s = y * inv;
     } else {
@@ -201,9 +203,11 @@ __device__ __forceinline__ void svd2(const Mat2& F, float& uc, float& us, float&
     const float off = 0.5f * (S01 + S10);
     float jc = 1.f, js = 0.f;
     if (off != 0.f) {
+        // t = tan of the Jacobi angle, |t| <= 1; the square-root arguments lie in [1, inf) and [1, 2], so
+        // the hardware square root and reciprocal square root (1 ulp) need no denormal scaling
         const float tau = 0.5f * (S11 - S00) * __builtin_amdgcn_rcpf(off);
-        const float t = copysignf(__builtin_amdgcn_rcpf(fabsf(tau) + sqrtf(1.f + tau * tau)), tau);
-        jc = rsqrtf(1.f + t * t);
+        const float t = copysignf(__builtin_amdgcn_rcpf(fabsf(tau) + __builtin_amdgcn_sqrtf(1.f + tau * tau)), tau);
+        jc = __builtin_amdgcn_rsqf(1.f + t * t);
         js = t * jc;
     }
     // V = [[jc, js], [-js, jc]]
@@ -225,7 +229,7 @@ __device__ __forceinline__ void snow_project(const DevParams& p, Mat2& F, float&
     const float lo = 1.f - p.theta_c, hi = 1.f + p.theta_s;
     const float n0 = fminf(fmaxf(s0, lo), hi);
     const float n1 = fminf(fmaxf(s1, lo), hi);
-    Jp *= (s0 * s1) / (n0 * n1);
+    Jp *= (s0 * s1) * __builtin_amdgcn_rcpf(n0 * n1);  // n0 n1 in [lo^2, hi^2], near 1
     // U = [[uc, -us], [us, uc]], V = [[vc, -vs], [vs, vc]]; F = U diag(n) V^T
     const float u00 = uc, u01 = -us, u10 = us, u11 = uc;
     const float v00 = vc, v01 = -vs, v10 = vs, v11 = vc;

@@ -83,7 +83,8 @@ typedef struct MpmParams {
     float mass, vol;   /* particle mass and rest volume (kMass, kRestVol)                                     */
     float tait_k, tait_gamma, p_min, p_max; /* fluid: p = k((1/J)^gamma - 1) clamped (constants.h:84-87)     */
     float mu, lambda;  /* Lame parameters of jelly and snow                                                   */
-    float theta_c, theta_s, hardening; /* snow: clamp [1-theta_c, 1+theta_s], mu,lambda *= exp(h(1-Jp))    */
+    float theta_c, theta_s, hardening; /* snow: clamp [1-theta_c, 1+theta_s] (0 <= theta_c < 1, theta_s >= 0),
+                                          mu,lambda *= exp(h(1-Jp))                                          */
     float j_min, j_max;/* clamp of the tracked J (MPM.cu:265)                                                 */
     int32_t substeps;  /* substeps per mpm_step tick (kSubsteps, constants.h:72)                              */
     int32_t capacity;  /* particle capacity over all scenes (kNumAtoms, constants.h:83)                       */

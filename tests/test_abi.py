@@ -90,6 +90,10 @@ def test_create_rejects_bad_params_before_touching_a_device():
     bad = abi.preset(abi.PRESET_JELLY128)
     bad.sim_w = 2.0
     assert lib.mpm_create(ctypes.byref(bad), ctypes.byref(h)) == abi.MPM_ERR_INVALID_ARG
+    bad = abi.preset(abi.PRESET_MIXED1024)
+    bad.theta_c = 1.0  # singular values clamped to [0, ...]: Jp would divide by zero
+    assert lib.mpm_create(ctypes.byref(bad), ctypes.byref(h)) == abi.MPM_ERR_INVALID_ARG
+    assert b"theta_c" in lib.mpm_last_message(None)
     assert lib.mpm_create(None, ctypes.byref(h)) == abi.MPM_ERR_INVALID_ARG
 
 
