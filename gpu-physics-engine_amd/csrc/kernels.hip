@@ -750,32 +750,99 @@ __device__ __forceinline__ void zero_next_accumulators(const Grid& g)
     }
 }
 
-// LDS accumulators of P2G: one float4 (momentum x, momentum y, mass, unused) per tile node and per wave.
-// Each wave owns its tile, so no two waves ever update the same LDS word; the tiles are summed in a fixed
-// wave order at the flush.
-template <int BX, int BY>
-__device__ __forceinline__ void zero_acc_tiles(float4* acc)
-{
-    using T = TileShape<BX, BY>;
-    for (int t = threadIdx.x; t < T::N * kAccTiles; t += kThreads) { acc[t] = make_float4(0.f, 0.f, 0.f, 0.f); }
-}
-
-// One HBM atomic per touched tile node and component (the LDS tiles already reduced every particle of the
-// work item).
-template <int BX, int BY>
-__device__ __forceinline__ void flush_acc_tiles(const DevParams& p, const Grid& g, size_t gb, int ox, int oy,
-                                                const float4* acc)
-{
-    using T = TileShape<BX, BY>;
-    for (int t = threadIdx.x; t < T::N; t += kThreads) {
-        float4 v = acc[t];
+// LDS accumulators of P2G: per wave one tile of N nodes holding momentum x, momentum y and mass.  Each
+// wave owns its tile, so no two waves ever update the same LDS word; the tiles are summed in a fixed wave
+// order at the flush.  Two layouts:
+//  - PlaneTiles (plain kernels): momentum in a float2 plane and mass in a float plane, so a node update is
+//    one b64 and one b32 read-add-write.  Measured on gfx950 for a 9-node stencil of 64 lanes on distinct
+//    start nodes (4 waves per SIMD): 163 CU cycles per wave, where float4 nodes updated with b96 accesses
+//    take 244 (start nodes in row order) to 295 (checkerboard order) and with b128 accesses 176 to 225.
+//  - QuadTiles (learned kernels): one float4 per node; its fourth word holds the claims of the claimed
+//    rounds of scatter.
+template <int N>
+struct PlaneTiles {
+    float2 mom[N * kAccTiles];
+    float mass[N * kAccTiles];
+    struct Wave {
+        static constexpr bool kClaim = false;
+        float2* mom;
+        float* mass;
+        __device__ __forceinline__ void add(int node, float x, float y, float m) const
+        {
+            float2 v = mom[node];
+            v.x += x;
+            v.y += y;
+            mom[node] = v;
+            mass[node] += m;
+        }
+    };
+    __device__ __forceinline__ Wave wave(int w) { return {mom + w * N, mass + w * N}; }
+    __device__ __forceinline__ void zero(int t)
+    {
+        mom[t] = make_float2(0.f, 0.f);
+        mass[t] = 0.f;
+    }
+    // (momentum x, momentum y, mass) of node t summed over the waves in wave order
+    __device__ __forceinline__ float3 sum(int t) const
+    {
+        float3 v = make_float3(mom[t].x, mom[t].y, mass[t]);
 #pragma unroll
         for (int w = 1; w < kAccTiles; ++w) {
-            const float4 u = acc[w * T::N + t];
+            v.x += mom[w * N + t].x;
+            v.y += mom[w * N + t].y;
+            v.z += mass[w * N + t];
+        }
+        return v;
+    }
+};
+template <int N>
+struct QuadTiles {
+    float4 acc[N * kAccTiles];
+    struct Wave {
+        static constexpr bool kClaim = true;
+        float4* t;
+        __device__ __forceinline__ void add(int node, float x, float y, float m) const
+        {
+            float4 v = t[node];
+            v.x += x;
+            v.y += y;
+            v.z += m;
+            t[node] = v;
+        }
+        __device__ __forceinline__ float* claim(int node) const { return &t[node].w; }
+    };
+    __device__ __forceinline__ Wave wave(int w) { return {acc + w * N}; }
+    __device__ __forceinline__ void zero(int t) { acc[t] = make_float4(0.f, 0.f, 0.f, 0.f); }
+    __device__ __forceinline__ float3 sum(int t) const
+    {
+        float3 v = make_float3(acc[t].x, acc[t].y, acc[t].z);
+#pragma unroll
+        for (int w = 1; w < kAccTiles; ++w) {
+            const float4 u = acc[w * N + t];
             v.x += u.x;
             v.y += u.y;
             v.z += u.z;
         }
+        return v;
+    }
+};
+
+template <int BX, int BY, class Tiles>
+__device__ __forceinline__ void zero_acc_tiles(Tiles& acc)
+{
+    using T = TileShape<BX, BY>;
+    for (int t = threadIdx.x; t < T::N * kAccTiles; t += kThreads) { acc.zero(t); }
+}
+
+// One HBM atomic per touched tile node and component (the LDS tiles already reduced every particle of the
+// work item).
+template <int BX, int BY, class Tiles>
+__device__ __forceinline__ void flush_acc_tiles(const DevParams& p, const Grid& g, size_t gb, int ox, int oy,
+                                                const Tiles& acc)
+{
+    using T = TileShape<BX, BY>;
+    for (int t = threadIdx.x; t < T::N; t += kThreads) {
+        const float3 v = acc.sum(t);
         if (v.z == 0.f && v.x == 0.f && v.y == 0.f) { continue; }
         const int ty = t / T::TX;
         const int tx = t - ty * T::TX;
@@ -899,13 +966,14 @@ struct CeilLog2<1> {
 // previous write of the same wave; LDS executes a wave's instructions in order).  Two ways to form rounds:
 //  - ranked (default): lanes are ranked among those starting at the same node by an exact multisplit over
 //    the node index (one ballot per bit); round r takes rank r, so the summation order is the particle order;
-//  - claimed (kClaim, for the VALU-bound learned kernels): each pending lane writes its lane index into the
-//    unused fourth word of its start node and the lane that reads its own index back updates; the others
+//  - claimed (QuadTiles, for the VALU-bound learned kernels): each pending lane writes its lane index into
+//    the fourth word of its start node and the lane that reads its own index back updates; the others
 //    retry.  No ballots per bit, two more LDS instructions per round.
 // A particle that drifted out of the tile scatters to HBM atomically.
-template <int BX, int BY, bool kClaim = false>
-__device__ __forceinline__ void scatter(const DevParams& p, const Grid& g, size_t gb, int ox, int oy, float4* wtile,
-                                        float px, float py, float vx, float vy, const Mat2& Q, bool active = true)
+template <int BX, int BY, class WaveTile>
+__device__ __forceinline__ void scatter(const DevParams& p, const Grid& g, size_t gb, int ox, int oy,
+                                        const WaveTile& wtile, float px, float py, float vx, float vy, const Mat2& Q,
+                                        bool active = true)
 {
     using T = TileShape<BX, BY>;
     int bx, by;
@@ -953,11 +1021,11 @@ __device__ __forceinline__ void scatter(const DevParams& p, const Grid& g, size_
     }
     const int base = ly * T::TX + lx;
     const int lane = threadIdx.x & 63;
-    if constexpr (kClaim) {
+    if constexpr (WaveTile::kClaim) {
         // one claimant per start node reads its own index back (the barrier keeps the read after the write)
         bool pending = inside;
         while (__ballot(pending) != 0ull) {
-            float* claim = &wtile[base].w;
+            float* claim = wtile.claim(base);
             if (pending) { *claim = __int_as_float(lane); }
             __asm__ __volatile__("" ::: "memory");
             const bool won = pending && __float_as_int(*claim) == lane;
@@ -967,12 +1035,7 @@ __device__ __forceinline__ void scatter(const DevParams& p, const Grid& g, size_
                 for (int a = 0; a < 3; ++a) {
 #pragma unroll
                     for (int b = 0; b < 3; ++b) {
-                        float4* cell = &wtile[base + b * T::TX + a];
-                        float4 v = *cell;
-                        v.x += cx[a * 3 + b];
-                        v.y += cy[a * 3 + b];
-                        v.z += cm[a * 3 + b];
-                        *cell = v;
+                        wtile.add(base + b * T::TX + a, cx[a * 3 + b], cy[a * 3 + b], cm[a * 3 + b]);
                         __asm__ __volatile__("" ::: "memory");
                     }
                 }
@@ -996,12 +1059,7 @@ __device__ __forceinline__ void scatter(const DevParams& p, const Grid& g, size_
                 for (int a = 0; a < 3; ++a) {
 #pragma unroll
                     for (int b = 0; b < 3; ++b) {
-                        float4* cell = &wtile[base + b * T::TX + a];
-                        float4 v = *cell;
-                        v.x += cx[a * 3 + b];
-                        v.y += cy[a * 3 + b];
-                        v.z += cm[a * 3 + b];
-                        *cell = v;
+                        wtile.add(base + b * T::TX + a, cx[a * 3 + b], cy[a * 3 + b], cm[a * 3 + b]);
                         __asm__ __volatile__("" ::: "memory");
                     }
                 }
@@ -1017,7 +1075,7 @@ __global__ __launch_bounds__(kThreads) void k_p2g_tiled(DevParams p, Store s, Gr
                                                         const int32_t* num_items, const int32_t* count)
 {
     using T = TileShape<BX, BY>;
-    __shared__ float4 acc[T::N * kAccTiles];
+    __shared__ PlaneTiles<T::N> acc;
     zero_next_accumulators(g);
     bool ok;
     const int it = xcd_item(num_items[0], ok);
@@ -1029,7 +1087,7 @@ __global__ __launch_bounds__(kThreads) void k_p2g_tiled(DevParams p, Store s, Gr
     const size_t gb = (size_t)scene * (size_t)p.G;
     zero_acc_tiles<BX, BY>(acc);
     __syncthreads();
-    float4* wtile = acc + (threadIdx.x >> 6) * T::N;
+    const auto wtile = acc.wave(threadIdx.x >> 6);
     for (int j = item.begin + threadIdx.x; j < item.end; j += kThreads) {
         const uint32_t o = (uint32_t)j * 4u;
         const Mat2 Q = affine_q(p, ldo(s.mat, (uint32_t)j), ld_C(s, o), ld_F(s, o), ldo(s.J, o));
@@ -1084,7 +1142,7 @@ __global__ __launch_bounds__(kThreads) void k_g2p2g(DevParams p, Store s, Grid g
 {
     using T = TileShape<BX, BY>;
     __shared__ float2 sv[T::N];
-    __shared__ float4 acc[T::N * kAccTiles];
+    __shared__ PlaneTiles<T::N> acc;
     zero_next_accumulators(g);
     bool ok;
     const int it = xcd_item(num_items[0], ok);
@@ -1097,7 +1155,7 @@ __global__ __launch_bounds__(kThreads) void k_g2p2g(DevParams p, Store s, Grid g
     load_vel_tile<BX, BY>(p, g, gb, ox, oy, sv);
     zero_acc_tiles<BX, BY>(acc);
     __syncthreads();
-    float4* wtile = acc + (threadIdx.x >> 6) * T::N;
+    const auto wtile = acc.wave(threadIdx.x >> 6);
     for (int j = item.begin + threadIdx.x; j < item.end; j += kThreads) {
         const uint32_t o = (uint32_t)j * 4u;
         float px = ldo(s.x, o), py = ldo(s.y, o), vx, vy;
@@ -1143,7 +1201,7 @@ __global__ __launch_bounds__(kThreads) void k_p2g_tiled_learned(DevParams p, Sto
                                                                 const int32_t* num_items, const int32_t* count)
 {
     using T = TileShape<BX, BY>;
-    __shared__ float4 acc[T::N * kAccTiles];
+    __shared__ QuadTiles<T::N> acc;
     __shared__ ClosurePacked cl;
     zero_next_accumulators(g);
     bool ok;
@@ -1158,14 +1216,14 @@ __global__ __launch_bounds__(kThreads) void k_p2g_tiled_learned(DevParams p, Sto
     zero_acc_tiles<BX, BY>(acc);
     closure_stage(p.cl, &cl);
     __syncthreads();
-    float4* wtile = acc + (threadIdx.x >> 6) * T::N;
+    const auto wtile = acc.wave(threadIdx.x >> 6);
     for (int base = item.begin; base < item.end; base += kThreads) {
         const int j = base + threadIdx.x;
         const bool valid = j < item.end;
         const uint32_t o = (uint32_t)(valid ? j : item.end - 1) * 4u;
         const int mat = ldo(s.mat, o >> 2);
         const Mat2 Q = affine_q_learned(p, &cl, mat, ld_C(s, o), ld_F(s, o), ldo(s.J, o));
-        scatter<BX, BY, true>(p, g, gb, ox, oy, wtile, ldo(s.x, o), ldo(s.y, o), ldo(s.vx, o), ldo(s.vy, o), Q, valid);
+        scatter<BX, BY>(p, g, gb, ox, oy, wtile, ldo(s.x, o), ldo(s.y, o), ldo(s.vx, o), ldo(s.vy, o), Q, valid);
     }
     __syncthreads();
     flush_acc_tiles<BX, BY>(p, g, gb, ox, oy, acc);
@@ -1179,7 +1237,7 @@ __global__ __launch_bounds__(kThreads) void k_g2p2g_learned(DevParams p, Store s
 {
     using T = TileShape<BX, BY>;
     __shared__ float2 sv[T::N];
-    __shared__ float4 acc[T::N * kAccTiles];
+    __shared__ QuadTiles<T::N> acc;
     __shared__ ClosurePacked cl;
     zero_next_accumulators(g);
     bool ok;
@@ -1195,7 +1253,7 @@ __global__ __launch_bounds__(kThreads) void k_g2p2g_learned(DevParams p, Store s
     zero_acc_tiles<BX, BY>(acc);
     closure_stage(p.cl, &cl);
     __syncthreads();
-    float4* wtile = acc + (threadIdx.x >> 6) * T::N;
+    const auto wtile = acc.wave(threadIdx.x >> 6);
     for (int base = item.begin; base < item.end; base += kThreads) {
         const int j = base + threadIdx.x;
         const bool valid = j < item.end;
@@ -1213,7 +1271,7 @@ __global__ __launch_bounds__(kThreads) void k_g2p2g_learned(DevParams p, Store s
             sto(s.J, o, J);
         }
         const Mat2 Q = affine_q_learned(p, &cl, mat, C, F, J);
-        scatter<BX, BY, true>(p, g, gb, ox, oy, wtile, px, py, vx, vy, Q, valid);
+        scatter<BX, BY>(p, g, gb, ox, oy, wtile, px, py, vx, vy, Q, valid);
     }
     __syncthreads();
     flush_acc_tiles<BX, BY>(p, g, gb, ox, oy, acc);
