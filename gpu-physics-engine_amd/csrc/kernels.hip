@@ -763,10 +763,12 @@ template <int N>
 struct PlaneTiles {
     float2 mom[N * kAccTiles];
     float mass[N * kAccTiles];
+    uint8_t owner[N * kAccTiles];  // scratch of the ranking in scatter (written before every read)
     struct Wave {
         static constexpr bool kClaim = false;
         float2* mom;
         float* mass;
+        uint8_t* owner;
         __device__ __forceinline__ void add(int node, float x, float y, float m) const
         {
             float2 v = mom[node];
@@ -776,7 +778,7 @@ struct PlaneTiles {
             mass[node] += m;
         }
     };
-    __device__ __forceinline__ Wave wave(int w) { return {mom + w * N, mass + w * N}; }
+    __device__ __forceinline__ Wave wave(int w) { return {mom + w * N, mass + w * N, owner + w * N}; }
     __device__ __forceinline__ void zero(int t)
     {
         mom[t] = make_float2(0.f, 0.f);
@@ -950,22 +952,14 @@ __device__ __forceinline__ void gather(const DevParams& p, const Grid& g, size_t
     B = {p.dx * (mx - fx * sx), p.dx * (nx - fy * sx), p.dx * (my - fx * sy), p.dx * (ny - fy * sy)};
 }
 
-// Bits needed to name a tile node.
-template <int N>
-struct CeilLog2 {
-    static constexpr int value = (N <= 1) ? 0 : 1 + CeilLog2<(N + 1) / 2>::value;
-};
-template <>
-struct CeilLog2<1> {
-    static constexpr int value = 0;
-};
-
 // Scatter of one particle into its wave's LDS accumulator tile, without atomics, in rounds in which the
 // updating lanes start their stencils at distinct nodes, so for each of the 9 offsets their targets are
 // distinct and a plain read-add-write is race free (the compiler barriers keep every read after the
 // previous write of the same wave; LDS executes a wave's instructions in order).  Two ways to form rounds:
-//  - ranked (default): lanes are ranked among those starting at the same node by an exact multisplit over
-//    the node index (one ballot per bit); round r takes rank r, so the summation order is the particle order;
+//  - ranked (PlaneTiles): lanes are ranked among those starting at the same node (rank = lanes below with
+//    the same start node, by an exact multisplit, one ballot per bit, over a 6-bit name of the node that a
+//    claim in an LDS byte plane provides; skipped when the claim shows every start node distinct); round r
+//    takes rank r, so the summation order is the particle order;
 //  - claimed (QuadTiles, for the VALU-bound learned kernels): each pending lane writes its lane index into
 //    the fourth word of its start node and the lane that reads its own index back updates; the others
 //    retry.  No ballots per bit, two more LDS instructions per round.
@@ -1043,16 +1037,28 @@ __device__ __forceinline__ void scatter(const DevParams& p, const Grid& g, size_
             }
         }
     } else {
-        constexpr int kBits = CeilLog2<T::N>::value;
-        const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-        uint64_t peers = __ballot(inside);
+        // Every lane writes its index into the owner byte of its start node and reads the byte back: lanes
+        // with the same start node read the same index (that of the lane whose write landed last), lanes
+        // with different start nodes different ones.  When no lane reads another's index all start nodes are
+        // distinct and every rank is 0; otherwise the index read back is an exact 6-bit name of the start
+        // node, and the multisplit runs over it.
+        uint8_t* own = wtile.owner + base;
+        if (inside) { *own = (uint8_t)lane; }
+        __asm__ __volatile__("" ::: "memory");
+        const int name = inside ? (int)*own : lane;
+        __asm__ __volatile__("" ::: "memory");
+        int rank = 0;
+        if (__ballot(name != lane) != 0ull) {
+            const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+            uint64_t peers = __ballot(inside);
 #pragma unroll
-        for (int k = 0; k < kBits; ++k) {
-            const bool bit = (base >> k) & 1;
-            const uint64_t m = __ballot(inside && bit);
-            peers &= bit ? m : ~m;
+            for (int k = 0; k < 6; ++k) {
+                const bool bit = (name >> k) & 1;
+                const uint64_t m = __ballot(inside && bit);
+                peers &= bit ? m : ~m;
+            }
+            rank = __popcll(peers & lt_mask);
         }
-        const int rank = __popcll(peers & lt_mask);
         for (int r = 0;; ++r) {
             if (inside && rank == r) {
 #pragma unroll
