@@ -764,19 +764,25 @@ struct PlaneTiles {
     float2 mom[N * kAccTiles];
     float mass[N * kAccTiles];
     uint8_t owner[N * kAccTiles];  // scratch of the ranking in scatter (written before every read)
+    // Nodes relative to a start node: the stencil offsets become immediate offsets of the LDS instructions.
+    struct Cursor {
+        float2* mom;
+        float* mass;
+        __device__ __forceinline__ void add(int k, float x, float y, float m) const
+        {
+            float2 v = mom[k];
+            v.x += x;
+            v.y += y;
+            mom[k] = v;
+            mass[k] += m;
+        }
+    };
     struct Wave {
         static constexpr bool kClaim = false;
         float2* mom;
         float* mass;
         uint8_t* owner;
-        __device__ __forceinline__ void add(int node, float x, float y, float m) const
-        {
-            float2 v = mom[node];
-            v.x += x;
-            v.y += y;
-            mom[node] = v;
-            mass[node] += m;
-        }
+        __device__ __forceinline__ Cursor at(int node) const { return {mom + node, mass + node}; }
     };
     __device__ __forceinline__ Wave wave(int w) { return {mom + w * N, mass + w * N, owner + w * N}; }
     __device__ __forceinline__ void zero(int t)
@@ -800,17 +806,21 @@ struct PlaneTiles {
 template <int N>
 struct QuadTiles {
     float4 acc[N * kAccTiles];
-    struct Wave {
-        static constexpr bool kClaim = true;
+    struct Cursor {
         float4* t;
-        __device__ __forceinline__ void add(int node, float x, float y, float m) const
+        __device__ __forceinline__ void add(int k, float x, float y, float m) const
         {
-            float4 v = t[node];
+            float4 v = t[k];
             v.x += x;
             v.y += y;
             v.z += m;
-            t[node] = v;
+            t[k] = v;
         }
+    };
+    struct Wave {
+        static constexpr bool kClaim = true;
+        float4* t;
+        __device__ __forceinline__ Cursor at(int node) const { return {t + node}; }
         __device__ __forceinline__ float* claim(int node) const { return &t[node].w; }
     };
     __device__ __forceinline__ Wave wave(int w) { return {acc + w * N}; }
@@ -1015,6 +1025,7 @@ __device__ __forceinline__ void scatter(const DevParams& p, const Grid& g, size_
     }
     const int base = ly * T::TX + lx;
     const int lane = threadIdx.x & 63;
+    const auto cur = wtile.at(base);
     if constexpr (WaveTile::kClaim) {
         // one claimant per start node reads its own index back (the barrier keeps the read after the write)
         bool pending = inside;
@@ -1029,7 +1040,7 @@ __device__ __forceinline__ void scatter(const DevParams& p, const Grid& g, size_
                 for (int a = 0; a < 3; ++a) {
 #pragma unroll
                     for (int b = 0; b < 3; ++b) {
-                        wtile.add(base + b * T::TX + a, cx[a * 3 + b], cy[a * 3 + b], cm[a * 3 + b]);
+                        cur.add(b * T::TX + a, cx[a * 3 + b], cy[a * 3 + b], cm[a * 3 + b]);
                         __asm__ __volatile__("" ::: "memory");
                     }
                 }
@@ -1049,15 +1060,21 @@ __device__ __forceinline__ void scatter(const DevParams& p, const Grid& g, size_
         __asm__ __volatile__("" ::: "memory");
         int rank = 0;
         if (__ballot(name != lane) != 0ull) {
-            const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-            uint64_t peers = __ballot(inside);
+            // peers: inside lanes whose name equals this lane's, bit by bit: with m = the lanes whose bit k
+            // is set and B = 0 when this lane's bit k is set, all ones otherwise, m ^ B selects m or ~m
+            // (the form takes one SGPR operand per VALU instruction)
+            const uint64_t in = __ballot(inside);
+            uint32_t plo = (uint32_t)in, phi = (uint32_t)(in >> 32);
+            const int nn = ~name;
 #pragma unroll
             for (int k = 0; k < 6; ++k) {
-                const bool bit = (name >> k) & 1;
-                const uint64_t m = __ballot(inside && bit);
-                peers &= bit ? m : ~m;
+                const int B = (nn << (31 - k)) >> 31;
+                const uint64_t m = __ballot(B == 0);
+                plo &= (uint32_t)m ^ (uint32_t)B;
+                phi &= (uint32_t)(m >> 32) ^ (uint32_t)B;
             }
-            rank = __popcll(peers & lt_mask);
+            const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+            rank = __popcll((((uint64_t)phi << 32) | plo) & lt_mask);
         }
         for (int r = 0;; ++r) {
             if (inside && rank == r) {
@@ -1065,7 +1082,7 @@ __device__ __forceinline__ void scatter(const DevParams& p, const Grid& g, size_
                 for (int a = 0; a < 3; ++a) {
 #pragma unroll
                     for (int b = 0; b < 3; ++b) {
-                        wtile.add(base + b * T::TX + a, cx[a * 3 + b], cy[a * 3 + b], cm[a * 3 + b]);
+                        cur.add(b * T::TX + a, cx[a * 3 + b], cy[a * 3 + b], cm[a * 3 + b]);
                         __asm__ __volatile__("" ::: "memory");
                     }
                 }
