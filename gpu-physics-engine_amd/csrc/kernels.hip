@@ -1059,7 +1059,8 @@ __device__ __forceinline__ void scatter(const DevParams& p, const Grid& g, size_
         const int name = inside ? (int)*own : lane;
         __asm__ __volatile__("" ::: "memory");
         int rank = 0;
-        if (__ballot(name != lane) != 0ull) {
+        const bool conflict = __ballot(name != lane) != 0ull;  // wave-uniform
+        if (conflict) {
             // peers: inside lanes whose name equals this lane's, bit by bit: with m = the lanes whose bit k
             // is set and B = 0 when this lane's bit k is set, all ones otherwise, m ^ B selects m or ~m
             // (the form takes one SGPR operand per VALU instruction)
@@ -1087,7 +1088,7 @@ __device__ __forceinline__ void scatter(const DevParams& p, const Grid& g, size_
                     }
                 }
             }
-            if (__ballot(inside && rank > r) == 0ull) { break; }
+            if (!conflict || __ballot(inside && rank > r) == 0ull) { break; }
         }
     }
 }

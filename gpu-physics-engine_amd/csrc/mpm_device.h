@@ -267,33 +267,26 @@ __device__ __forceinline__ Mat2 affine_q(const DevParams& p, int mat, const Mat2
     return {p.mass * C.a00 - D * t00, p.mass * C.a01 - D * t01, p.mass * C.a10 - D * t01, p.mass * C.a11 - D * t11};
 }
 
-// handleOOB (MPM.cu:267-301).
+// handleOOB (MPM.cu:267-301), one axis: a non-finite position or velocity component resets the position to
+// the middle of the domain and the velocity to 0; then x < 0 sets x = 0, v = max(v, 0) and x > w sets x = w,
+// v = min(v, 0) (w > 0, so at most one applies).  Written as two med3 clamps: v is finite there, and
+// med3(v, 0, +inf) = max(v, 0), med3(v, -inf, 0) = min(v, 0), med3(x, 0, w) = the clamped x (up to the sign
+// of a zero result).
+__device__ __forceinline__ void oob_axis(float& x, float& v, float w)
+{
+    if (!isfinite(x) || !isfinite(v)) {
+        x = 0.5f * w;
+        v = 0.f;
+    }
+    const float lo = x < 0.f ? 0.f : -INFINITY;
+    const float hi = x > w ? 0.f : INFINITY;
+    v = __builtin_amdgcn_fmed3f(v, lo, hi);
+    x = __builtin_amdgcn_fmed3f(x, 0.f, w);
+}
 __device__ __forceinline__ void handle_oob(const DevParams& p, float& px, float& py, float& vx, float& vy)
 {
-    if (!isfinite(px) || !isfinite(vx)) {
-        px = 0.5f * p.sim_w;
-        vx = 0.f;
-    }
-    if (!isfinite(py) || !isfinite(vy)) {
-        py = 0.5f * p.sim_h;
-        vy = 0.f;
-    }
-    if (px < 0.f) {
-        px = 0.f;
-        vx = fmaxf(vx, 0.f);
-    }
-    if (px > p.sim_w) {
-        px = p.sim_w;
-        vx = fminf(vx, 0.f);
-    }
-    if (py < 0.f) {
-        py = 0.f;
-        vy = fmaxf(vy, 0.f);
-    }
-    if (py > p.sim_h) {
-        py = p.sim_h;
-        vy = fminf(vy, 0.f);
-    }
+    oob_axis(px, vx, p.sim_w);
+    oob_axis(py, vy, p.sim_h);
 }
 
 // Particle update after the gather (MPM.cu:252-265): advect, C = 4 inv_dx^2 B (before the OOB handling),
