@@ -750,7 +750,8 @@ __device__ __forceinline__ void zero_next_accumulators(const Grid& g)
     }
 }
 
-// LDS accumulators of P2G: per wave one tile of N nodes holding momentum x, momentum y and mass.  Each
+// LDS accumulators of P2G: per wave one tile of N nodes holding momentum x, momentum y and the weight sum
+// (mass / particle mass: every particle has the same mass, applied once per node at the flush).  Each
 // wave owns its tile, so no two waves ever update the same LDS word; the tiles are summed in a fixed wave
 // order at the flush.  Two layouts:
 //  - PlaneTiles (plain kernels): momentum in a float2 plane and mass in a float plane, so a node update is
@@ -768,13 +769,16 @@ struct PlaneTiles {
     struct Cursor {
         float2* mom;
         float* mass;
+        // both planes are read before either is written (one LDS round trip per node, not two)
         __device__ __forceinline__ void add(int k, float x, float y, float m) const
         {
             float2 v = mom[k];
+            float w = mass[k];
             v.x += x;
             v.y += y;
+            w += m;
             mom[k] = v;
-            mass[k] += m;
+            mass[k] = w;
         }
     };
     struct Wave {
@@ -863,7 +867,7 @@ __device__ __forceinline__ void flush_acc_tiles(const DevParams& p, const Grid& 
         const size_t node = gb + (size_t)gy * p.W + gx;
         atomic_add_global(&g.mom_x[node], v.x);
         atomic_add_global(&g.mom_y[node], v.y);
-        atomic_add_global(&g.mass[node], v.z);
+        atomic_add_global(&g.mass[node], v.z * p.mass);
     }
 }
 
@@ -993,7 +997,8 @@ __device__ __forceinline__ void scatter(const DevParams& p, const Grid& g, size_
     const float qax = p.dx * Q.a00, qay = p.dx * Q.a10, qbx = p.dx * Q.a01, qby = p.dx * Q.a11;
     const float a0x = vx * p.mass - (qax * fx + qbx * fy);
     const float a0y = vy * p.mass - (qay * fx + qby * fy);
-    const float wym[3] = {wy[0] * p.mass, wy[1] * p.mass, wy[2] * p.mass};
+    // The mass contribution is m w_ab with one mass for every particle: the tiles accumulate w_ab and the
+    // flush multiplies by m once per node.
     float cx[9], cy[9], cm[9];
     float tax = a0x, tay = a0y;
 #pragma unroll
@@ -1004,7 +1009,7 @@ __device__ __forceinline__ void scatter(const DevParams& p, const Grid& g, size_
             const float w = wx[a] * wy[b];
             cx[a * 3 + b] = w * tx;
             cy[a * 3 + b] = w * ty;
-            cm[a * 3 + b] = wx[a] * wym[b];
+            cm[a * 3 + b] = w;
             tx += qbx;
             ty += qby;
         }
@@ -1019,7 +1024,7 @@ __device__ __forceinline__ void scatter(const DevParams& p, const Grid& g, size_
                 const size_t node = gb + (size_t)(by + b) * p.W + (bx + a);
                 atomic_add_global(&g.mom_x[node], cx[a * 3 + b]);
                 atomic_add_global(&g.mom_y[node], cy[a * 3 + b]);
-                atomic_add_global(&g.mass[node], cm[a * 3 + b]);
+                atomic_add_global(&g.mass[node], cm[a * 3 + b] * p.mass);
             }
         }
     }
