@@ -1166,8 +1166,10 @@ __global__ __launch_bounds__(kThreads) void k_g2p_tiled(DevParams p, Store s, Gr
 // s+1, flushed with one HBM atomic per touched node.  v and C stay in registers (they are recomputed
 // by every G2P); they are written only by the last G2P of a segment (k_g2p_tiled).
 template <int BX, int BY>
-__global__ __launch_bounds__(kThreads) void k_g2p2g(DevParams p, Store s, Grid g, const Item* items,
-                                                    const int32_t* num_items, const int32_t* count)
+// At least 7 waves per SIMD (at most 72 VGPRs): more resident waves hide the latency of the scatter's
+// dependent LDS read-add-write chain (8 would spill to scratch).
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7))) void k_g2p2g(
+    DevParams p, Store s, Grid g, const Item* items, const int32_t* num_items, const int32_t* count)
 {
     using T = TileShape<BX, BY>;
     __shared__ float2 sv[T::N];
