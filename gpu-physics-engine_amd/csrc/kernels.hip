@@ -469,9 +469,14 @@ __global__ __launch_bounds__(kThreads) void k_radix_downsweep(const uint32_t* ke
 }
 
 // Permutation gather of the whole particle state: dst[j] = src[perm[j]] (keys already in dst.key).
+// Work groups are dispatched round-robin over the 8 XCDs; the grid is a multiple of 8 and XCD x takes the
+// x-th contiguous eighth of the destination, so the source lines of a bin (whose particles stay close to
+// their previous positions) are fetched into one XCD's L2 rather than into all eight.
 __global__ __launch_bounds__(kThreads) void k_gather(Store src, Store dst, const int32_t* perm, const int32_t* count)
 {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned per = gridDim.x >> 3;
+    const unsigned lb = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+    const int j = (int)(lb * blockDim.x + threadIdx.x);
     const int n = count[0];
     if (j >= n) { return; }
     const int i = perm[j];
@@ -671,7 +676,7 @@ hipError_t launch_binning(hipStream_t st, const DevParams& p, const Store& src, 
                        ss.bin_start, ss.bin_end);
     hipLaunchKernelGGL(k_layer_bins, dim3((unsigned)ss.num_keys), dim3(kThreads), 0, st, kin, vin, ss.bin_start,
                        ss.bin_end, ss.num_keys, p.node_bits, p.bin_x, dst.key, ss.perm);
-    hipLaunchKernelGGL(k_gather, dim3(pblocks), dim3(kThreads), 0, st, src, dst, ss.perm, count);
+    hipLaunchKernelGGL(k_gather, dim3((pblocks + 7u) / 8u * 8u), dim3(kThreads), 0, st, src, dst, ss.perm, count);
     const unsigned kblocks = (unsigned)((ss.num_keys + kThreads - 1) / kThreads);
     hipLaunchKernelGGL(k_item_counts, dim3(kblocks), dim3(kThreads), 0, st, ss.bin_start, ss.bin_end, ss.num_keys,
                        ss.chunk, ss.item_cnt);
