@@ -1079,7 +1079,7 @@ __device__ __forceinline__ void scatter(const DevParams& p, const Grid& g, size_
             const int nn = ~name;
 #pragma unroll
             for (int k = 0; k < 6; ++k) {
-                const int B = (nn << (31 - k)) >> 31;
+                const int B = __builtin_amdgcn_sbfe(nn, k, 1);  // bit k of ~name, sign-extended: 0 or -1
                 const uint64_t m = __ballot(B == 0);
                 plo &= (uint32_t)m ^ (uint32_t)B;
                 phi &= (uint32_t)(m >> 32) ^ (uint32_t)B;

@@ -95,6 +95,30 @@ def test_mixed_materials_small(oracle_mod, mode, substeps):
 
 
 @pytest.mark.parametrize("mode", MODES, ids=lambda m: MODE_NAMES[m])
+def test_crowded_start_nodes(oracle_mod, mode):
+    """1000 particles inside one cell (whole waves start their stencils at one node: 64 scatter rounds, the
+    full 6-bit multisplit, more particles per node than binning layers), 800 in a 2 x 2-cell cluster and
+    1200 spread over a block, with random v and C, 10 substeps against the FP64 oracle."""
+    rng = np.random.default_rng(41)
+    cells = 64
+    one = (np.array([32.5, 40.5]) + rng.uniform(-0.3, 0.3, (1000, 2))) / cells
+    four = (np.array([20.0, 24.0]) + rng.uniform(-0.9, 0.9, (800, 2))) / cells
+    spread = rng.uniform(0.3, 0.7, (1200, 2))
+    x = np.concatenate([one, four, spread]).astype(np.float32)
+    n = x.shape[0]
+    v = rng.uniform(-0.5, 0.5, (n, 2)).astype(np.float32)
+    C = rng.uniform(-1, 1, (n, 4)).astype(np.float32)
+    mat = rng.integers(0, 3, n).astype(np.uint8)
+    p = small_params(substeps=10, mode=mode, rho=4.0)
+    out = run_gpu(p, x, v, mat, C=C)
+    s, grid = run_oracle(oracle_mod, p, x, v, mat, 10, threads=1, C=C)
+    assert rel(out["x"], s.x) < 1e-6
+    assert rel(out["v"], s.v) < 1e-4
+    assert rel(out["F"], s.F) < 1e-5
+    assert rel(out["grid"][0][0], grid[0]) < 1e-5
+
+
+@pytest.mark.parametrize("mode", MODES, ids=lambda m: MODE_NAMES[m])
 def test_jelly_block_100_substeps(oracle_mod, mode):
     """The 128-cell jelly block (32,768 particles) after 100 substeps against the FP64 oracle."""
     p = abi.preset(abi.PRESET_JELLY128)
