@@ -1084,8 +1084,7 @@ __device__ __forceinline__ void scatter(const DevParams& p, const Grid& g, size_
                 plo &= (uint32_t)m ^ (uint32_t)B;
                 phi &= (uint32_t)(m >> 32) ^ (uint32_t)B;
             }
-            const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-            rank = __popcll((((uint64_t)phi << 32) | plo) & lt_mask);
+            rank = (int)__builtin_amdgcn_mbcnt_hi(phi, __builtin_amdgcn_mbcnt_lo(plo, 0u));  // peers below
         }
         for (int r = 0;; ++r) {
             if (inside && rank == r) {
@@ -1121,7 +1120,7 @@ __global__ __launch_bounds__(kThreads) void k_p2g_tiled(DevParams p, Store s, Gr
     const size_t gb = (size_t)scene * (size_t)p.G;
     zero_acc_tiles<BX, BY>(acc);
     __syncthreads();
-    const auto wtile = acc.wave(threadIdx.x >> 6);
+    const auto wtile = acc.wave(__builtin_amdgcn_readfirstlane(threadIdx.x >> 6));  // wave-uniform
     for (int j = item.begin + threadIdx.x; j < item.end; j += kThreads) {
         const uint32_t o = (uint32_t)j * 4u;
         const Mat2 Q = affine_q(p, ldo(s.mat, (uint32_t)j), ld_C(s, o), ld_F(s, o), ldo(s.J, o));
@@ -1191,7 +1190,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7))) v
     load_vel_tile<BX, BY>(p, g, gb, ox, oy, sv);
     zero_acc_tiles<BX, BY>(acc);
     __syncthreads();
-    const auto wtile = acc.wave(threadIdx.x >> 6);
+    const auto wtile = acc.wave(__builtin_amdgcn_readfirstlane(threadIdx.x >> 6));  // wave-uniform
     for (int j = item.begin + threadIdx.x; j < item.end; j += kThreads) {
         const uint32_t o = (uint32_t)j * 4u;
         float px = ldo(s.x, o), py = ldo(s.y, o), vx, vy;
@@ -1252,7 +1251,7 @@ __global__ __launch_bounds__(kThreads) void k_p2g_tiled_learned(DevParams p, Sto
     zero_acc_tiles<BX, BY>(acc);
     closure_stage(p.cl, &cl);
     __syncthreads();
-    const auto wtile = acc.wave(threadIdx.x >> 6);
+    const auto wtile = acc.wave(__builtin_amdgcn_readfirstlane(threadIdx.x >> 6));  // wave-uniform
     for (int base = item.begin; base < item.end; base += kThreads) {
         const int j = base + threadIdx.x;
         const bool valid = j < item.end;
@@ -1289,7 +1288,7 @@ __global__ __launch_bounds__(kThreads) void k_g2p2g_learned(DevParams p, Store s
     zero_acc_tiles<BX, BY>(acc);
     closure_stage(p.cl, &cl);
     __syncthreads();
-    const auto wtile = acc.wave(threadIdx.x >> 6);
+    const auto wtile = acc.wave(__builtin_amdgcn_readfirstlane(threadIdx.x >> 6));  // wave-uniform
     for (int base = item.begin; base < item.end; base += kThreads) {
         const int j = base + threadIdx.x;
         const bool valid = j < item.end;
