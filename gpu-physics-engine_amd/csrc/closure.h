@@ -89,7 +89,8 @@ struct ClosurePacked {
     // with c = -log2(e), W1s = c W1 split as W1s = W1hi + W1lo (BF16 parts):
     bf16x8 a1[2 * 64];  // layer 1, M-tile mt: A[u][8h+j] = (h ? W1lo : W1hi)[u][j % 4], u = 32 mt + l%32
     bf16x8 a2[8 * 64];  // layer 2, (mt, ks): A[m][8h+j] = W2[32 mt + l%32][chain_unit(ks, h, j)]
-    bf16x8 a3[4 * 64];  // layer 3, ks: A[m][8h+j] = m < 3 ? -ln 2 W3[m][chain_unit(ks, h, j)] : 0
+    bf16x8 a3[4 * 8];   // layer 3, ks: [8 ks + 4 h + m] = A[m][8h+j] = -ln 2 W3[m][chain_unit(ks, h, j)] for the
+                        // three output rows m < 3, zero for m = 3; rows 3..31 of A are zero and read entry m = 3
     float b1p[2][32];   // [h][16 mt + r] = c b1[32 mt + 8 (r/4) + 4 h + r%4]
     float b2p[2][32];   // likewise c b2
     float b3[4];
@@ -184,7 +185,8 @@ __device__ __forceinline__ void closure_eval_wave(const ClosurePacked* s, const 
         floatx16 out = {};
 #pragma unroll
         for (int ks = 0; ks < 4; ++ks) {
-            out = __builtin_amdgcn_mfma_f32_32x32x16_bf16(s->a3[ks * 64 + lane], b3[ks], out, 0, 0, 0);
+            out = __builtin_amdgcn_mfma_f32_32x32x16_bf16(s->a3[ks * 8 + half * 4 + min(lane & 31, 3)], b3[ks], out,
+                                                          0, 0, 0);
         }
         // rows 0..2 (the three outputs) sit in registers 0..2 of the lower half-wave
         res[nt][0] = out[0];

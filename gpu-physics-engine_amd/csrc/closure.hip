@@ -246,17 +246,19 @@ __global__ void k_pack_closure(const float* w, ClosurePacked* out)
         const int blk = e / 64 - 2, mt = blk / 4, ks = blk % 4;
         for (int j = 0; j < 8; ++j) { v[j] = (__bf16)w[kOffW2 + kHid * (32 * mt + m) + chain_unit(ks, h, j)]; }
         out->a2[e - 2 * 64] = v;
-    } else if (e < 14 * 64) {
-        const int ks = e / 64 - 10;
-        for (int j = 0; j < 8; ++j) { v[j] = (__bf16)(m < 3 ? kL * w[kOffW3 + kHid * m + chain_unit(ks, h, j)] : 0.f); }
-        out->a3[e - 10 * 64] = v;
-    } else if (e < 14 * 64 + 128) {
+    } else if (e < 10 * 64 + 32) {
+        const int q = e - 10 * 64, ks = q / 8, hq = (q / 4) % 2, mq = q % 4;
+        for (int j = 0; j < 8; ++j) {
+            v[j] = (__bf16)(mq < 3 ? kL * w[kOffW3 + kHid * mq + chain_unit(ks, hq, j)] : 0.f);
+        }
+        out->a3[q] = v;
+    } else if (e < 10 * 64 + 32 + 128) {
         // bias of C register r of M-tile mt in half-wave hh
-        const int b = e - 14 * 64, layer = b / 64, hh = (b / 32) % 2, mt = (b / 16) % 2, r = b % 16;
+        const int b = e - (10 * 64 + 32), layer = b / 64, hh = (b / 32) % 2, mt = (b / 16) % 2, r = b % 16;
         const float bias = kC * w[(layer ? kOffB2 : kOffB1) + 32 * mt + 8 * (r / 4) + 4 * hh + r % 4];
         (layer ? out->b2p : out->b1p)[hh][16 * mt + r] = bias;
-    } else if (e < 14 * 64 + 132) {
-        const int o = e - 14 * 64 - 128;
+    } else if (e < 10 * 64 + 32 + 132) {
+        const int o = e - (10 * 64 + 32) - 128;
         out->b3[o] = o < 3 ? w[kOffB3 + o] : 0.f;
     }
 }
@@ -281,7 +283,7 @@ size_t closure_packed_bytes() { return sizeof(ClosurePacked); }
 
 hipError_t launch_pack_closure(hipStream_t st, const float* w, void* out)
 {
-    hipLaunchKernelGGL(k_pack_closure, dim3((14 * 64 + 132 + kThreads - 1) / kThreads), dim3(kThreads), 0, st, w,
+    hipLaunchKernelGGL(k_pack_closure, dim3((10 * 64 + 32 + 132 + kThreads - 1) / kThreads), dim3(kThreads), 0, st, w,
                        reinterpret_cast<ClosurePacked*>(out));
     return hipGetLastError();
 }

@@ -758,13 +758,11 @@ __device__ __forceinline__ void zero_next_accumulators(const Grid& g)
 // LDS accumulators of P2G: per wave one tile of N nodes holding momentum x, momentum y and the weight sum
 // (mass / particle mass: every particle has the same mass, applied once per node at the flush).  Each
 // wave owns its tile, so no two waves ever update the same LDS word; the tiles are summed in a fixed wave
-// order at the flush.  Two layouts:
-//  - PlaneTiles (plain kernels): momentum in a float2 plane and mass in a float plane, so a node update is
-//    one b64 and one b32 read-add-write.  Measured on gfx950 for a 9-node stencil of 64 lanes on distinct
-//    start nodes (4 waves per SIMD): 163 CU cycles per wave, where float4 nodes updated with b96 accesses
-//    take 244 (start nodes in row order) to 295 (checkerboard order) and with b128 accesses 176 to 225.
-//  - QuadTiles (learned kernels): one float4 per node; its fourth word holds the claims of the claimed
-//    rounds of scatter.
+// order at the flush.  Layout (PlaneTiles): momentum in a float2 plane and the weight sum in a float plane,
+// so a node update is one b64 and one b32 read-add-write.  Measured on gfx950 for a 9-node stencil of 64
+// lanes on distinct start nodes (4 waves per SIMD): 163 CU cycles per wave, where float4 nodes updated
+// with b96 accesses take 244 (start nodes in row order) to 295 (checkerboard order) and with b128 accesses
+// 176 to 225.  A byte plane holds the start-node claims of the scatter's ranking.
 template <int N>
 struct PlaneTiles {
     float2 mom[N * kAccTiles];
@@ -787,7 +785,6 @@ struct PlaneTiles {
         }
     };
     struct Wave {
-        static constexpr bool kClaim = false;
         float2* mom;
         float* mass;
         uint8_t* owner;
@@ -812,42 +809,6 @@ struct PlaneTiles {
         return v;
     }
 };
-template <int N>
-struct QuadTiles {
-    float4 acc[N * kAccTiles];
-    struct Cursor {
-        float4* t;
-        __device__ __forceinline__ void add(int k, float x, float y, float m) const
-        {
-            float4 v = t[k];
-            v.x += x;
-            v.y += y;
-            v.z += m;
-            t[k] = v;
-        }
-    };
-    struct Wave {
-        static constexpr bool kClaim = true;
-        float4* t;
-        __device__ __forceinline__ Cursor at(int node) const { return {t + node}; }
-        __device__ __forceinline__ float* claim(int node) const { return &t[node].w; }
-    };
-    __device__ __forceinline__ Wave wave(int w) { return {acc + w * N}; }
-    __device__ __forceinline__ void zero(int t) { acc[t] = make_float4(0.f, 0.f, 0.f, 0.f); }
-    __device__ __forceinline__ float3 sum(int t) const
-    {
-        float3 v = make_float3(acc[t].x, acc[t].y, acc[t].z);
-#pragma unroll
-        for (int w = 1; w < kAccTiles; ++w) {
-            const float4 u = acc[w * N + t];
-            v.x += u.x;
-            v.y += u.y;
-            v.z += u.z;
-        }
-        return v;
-    }
-};
-
 template <int BX, int BY, class Tiles>
 __device__ __forceinline__ void zero_acc_tiles(Tiles& acc)
 {
@@ -974,15 +935,11 @@ __device__ __forceinline__ void gather(const DevParams& p, const Grid& g, size_t
 // Scatter of one particle into its wave's LDS accumulator tile, without atomics, in rounds in which the
 // updating lanes start their stencils at distinct nodes, so for each of the 9 offsets their targets are
 // distinct and a plain read-add-write is race free (the compiler barriers keep every read after the
-// previous write of the same wave; LDS executes a wave's instructions in order).  Two ways to form rounds:
-//  - ranked (PlaneTiles): lanes are ranked among those starting at the same node (rank = lanes below with
-//    the same start node, by an exact multisplit, one ballot per bit, over a 6-bit name of the node that a
-//    claim in an LDS byte plane provides; skipped when the claim shows every start node distinct); round r
-//    takes rank r, so the summation order is the particle order;
-//  - claimed (QuadTiles, for the VALU-bound learned kernels): each pending lane writes its lane index into
-//    the fourth word of its start node and the lane that reads its own index back updates; the others
-//    retry.  No ballots per bit, two more LDS instructions per round.
-// A particle that drifted out of the tile scatters to HBM atomically.
+// previous write of the same wave; LDS executes a wave's instructions in order).  Lanes are ranked among
+// those starting at the same node (rank = lanes below with the same start node, by an exact multisplit,
+// one ballot per bit, over a 6-bit name of the node that a claim in an LDS byte plane provides; skipped
+// when the claim shows every start node distinct); round r takes rank r, so the summation order is the
+// particle order.  A particle that drifted out of the tile scatters to HBM atomically.
 template <int BX, int BY, class WaveTile>
 __device__ __forceinline__ void scatter(const DevParams& p, const Grid& g, size_t gb, int ox, int oy,
                                         const WaveTile& wtile, float px, float py, float vx, float vy, const Mat2& Q,
@@ -1036,69 +993,46 @@ __device__ __forceinline__ void scatter(const DevParams& p, const Grid& g, size_
     const int base = ly * T::TX + lx;
     const int lane = threadIdx.x & 63;
     const auto cur = wtile.at(base);
-    if constexpr (WaveTile::kClaim) {
-        // one claimant per start node reads its own index back (the barrier keeps the read after the write)
-        bool pending = inside;
-        while (__ballot(pending) != 0ull) {
-            float* claim = wtile.claim(base);
-            if (pending) { *claim = __int_as_float(lane); }
-            __asm__ __volatile__("" ::: "memory");
-            const bool won = pending && __float_as_int(*claim) == lane;
-            __asm__ __volatile__("" ::: "memory");
-            if (won) {
+    // Every lane writes its index into the owner byte of its start node and reads the byte back: lanes
+    // with the same start node read the same index (that of the lane whose write landed last), lanes
+    // with different start nodes different ones.  When no lane reads another's index all start nodes are
+    // distinct and every rank is 0; otherwise the index read back is an exact 6-bit name of the start
+    // node, and the multisplit runs over it.
+    uint8_t* own = wtile.owner + base;
+    if (inside) { *own = (uint8_t)lane; }
+    __asm__ __volatile__("" ::: "memory");
+    const int name = inside ? (int)*own : lane;
+    __asm__ __volatile__("" ::: "memory");
+    int rank = 0;
+    const bool conflict = __ballot(name != lane) != 0ull;  // wave-uniform
+    if (conflict) {
+        // peers: inside lanes whose name equals this lane's, bit by bit: with m = the lanes whose bit k
+        // is set and B = 0 when this lane's bit k is set, all ones otherwise, m ^ B selects m or ~m
+        // (the form takes one SGPR operand per VALU instruction)
+        const uint64_t in = __ballot(inside);
+        uint32_t plo = (uint32_t)in, phi = (uint32_t)(in >> 32);
+        const int nn = ~name;
 #pragma unroll
-                for (int a = 0; a < 3; ++a) {
-#pragma unroll
-                    for (int b = 0; b < 3; ++b) {
-                        cur.add(b * T::TX + a, cx[a * 3 + b], cy[a * 3 + b], cm[a * 3 + b]);
-                        __asm__ __volatile__("" ::: "memory");
-                    }
-                }
-                pending = false;
-            }
+        for (int k = 0; k < 6; ++k) {
+            const int B = __builtin_amdgcn_sbfe(nn, k, 1);  // bit k of ~name, sign-extended: 0 or -1
+            const uint64_t m = __ballot(B == 0);
+            plo &= (uint32_t)m ^ (uint32_t)B;
+            phi &= (uint32_t)(m >> 32) ^ (uint32_t)B;
         }
-    } else {
-        // Every lane writes its index into the owner byte of its start node and reads the byte back: lanes
-        // with the same start node read the same index (that of the lane whose write landed last), lanes
-        // with different start nodes different ones.  When no lane reads another's index all start nodes are
-        // distinct and every rank is 0; otherwise the index read back is an exact 6-bit name of the start
-        // node, and the multisplit runs over it.
-        uint8_t* own = wtile.owner + base;
-        if (inside) { *own = (uint8_t)lane; }
-        __asm__ __volatile__("" ::: "memory");
-        const int name = inside ? (int)*own : lane;
-        __asm__ __volatile__("" ::: "memory");
-        int rank = 0;
-        const bool conflict = __ballot(name != lane) != 0ull;  // wave-uniform
-        if (conflict) {
-            // peers: inside lanes whose name equals this lane's, bit by bit: with m = the lanes whose bit k
-            // is set and B = 0 when this lane's bit k is set, all ones otherwise, m ^ B selects m or ~m
-            // (the form takes one SGPR operand per VALU instruction)
-            const uint64_t in = __ballot(inside);
-            uint32_t plo = (uint32_t)in, phi = (uint32_t)(in >> 32);
-            const int nn = ~name;
+        rank = (int)__builtin_amdgcn_mbcnt_hi(phi, __builtin_amdgcn_mbcnt_lo(plo, 0u));  // peers below
+    }
+    for (int r = 0;; ++r) {
+        if (inside && rank == r) {
 #pragma unroll
-            for (int k = 0; k < 6; ++k) {
-                const int B = __builtin_amdgcn_sbfe(nn, k, 1);  // bit k of ~name, sign-extended: 0 or -1
-                const uint64_t m = __ballot(B == 0);
-                plo &= (uint32_t)m ^ (uint32_t)B;
-                phi &= (uint32_t)(m >> 32) ^ (uint32_t)B;
-            }
-            rank = (int)__builtin_amdgcn_mbcnt_hi(phi, __builtin_amdgcn_mbcnt_lo(plo, 0u));  // peers below
-        }
-        for (int r = 0;; ++r) {
-            if (inside && rank == r) {
+            for (int a = 0; a < 3; ++a) {
 #pragma unroll
-                for (int a = 0; a < 3; ++a) {
-#pragma unroll
-                    for (int b = 0; b < 3; ++b) {
-                        cur.add(b * T::TX + a, cx[a * 3 + b], cy[a * 3 + b], cm[a * 3 + b]);
-                        __asm__ __volatile__("" ::: "memory");
-                    }
+                for (int b = 0; b < 3; ++b) {
+                    cur.add(b * T::TX + a, cx[a * 3 + b], cy[a * 3 + b], cm[a * 3 + b]);
+                    __asm__ __volatile__("" ::: "memory");
                 }
             }
-            if (!conflict || __ballot(inside && rank > r) == 0ull) { break; }
         }
+        if (!conflict || __ballot(inside && rank > r) == 0ull) { break; }
     }
 }
 
@@ -1232,11 +1166,11 @@ __device__ __forceinline__ Mat2 affine_q_learned(const DevParams& p, const Closu
 
 // k_p2g_tiled with the learned closure (uniform trip count so every wave runs the network with 64 lanes).
 template <int BX, int BY>
-__global__ __launch_bounds__(kThreads) void k_p2g_tiled_learned(DevParams p, Store s, Grid g, const Item* items,
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5))) void k_p2g_tiled_learned(DevParams p, Store s, Grid g, const Item* items,
                                                                 const int32_t* num_items, const int32_t* count)
 {
     using T = TileShape<BX, BY>;
-    __shared__ QuadTiles<T::N> acc;
+    __shared__ PlaneTiles<T::N> acc;
     __shared__ ClosurePacked cl;
     zero_next_accumulators(g);
     bool ok;
@@ -1267,12 +1201,12 @@ __global__ __launch_bounds__(kThreads) void k_p2g_tiled_learned(DevParams p, Sto
 // k_g2p2g with the learned closure: the P2G stress of every wave comes from one BF16 MFMA evaluation of
 // the network for its 64 particles.
 template <int BX, int BY>
-__global__ __launch_bounds__(kThreads) void k_g2p2g_learned(DevParams p, Store s, Grid g, const Item* items,
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5))) void k_g2p2g_learned(DevParams p, Store s, Grid g, const Item* items,
                                                             const int32_t* num_items, const int32_t* count)
 {
     using T = TileShape<BX, BY>;
     __shared__ float2 sv[T::N];
-    __shared__ QuadTiles<T::N> acc;
+    __shared__ PlaneTiles<T::N> acc;
     __shared__ ClosurePacked cl;
     zero_next_accumulators(g);
     bool ok;
