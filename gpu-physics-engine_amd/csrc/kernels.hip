@@ -1125,9 +1125,21 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7))) v
     zero_acc_tiles<BX, BY>(acc);
     __syncthreads();
     const auto wtile = acc.wave(__builtin_amdgcn_readfirstlane(threadIdx.x >> 6));  // wave-uniform
-    for (int j = item.begin + threadIdx.x; j < item.end; j += kThreads) {
+    // The position of the lane's next particle is loaded one iteration ahead (the gather needs it first;
+    // F, J and the material are needed later and keep their loads in the iteration).
+    int j = item.begin + threadIdx.x;
+    float nx = 0.f, ny = 0.f;
+    if (j < item.end) {
+        nx = ldo(s.x, (uint32_t)j * 4u);
+        ny = ldo(s.y, (uint32_t)j * 4u);
+    }
+    for (; j < item.end; j += kThreads) {
         const uint32_t o = (uint32_t)j * 4u;
-        float px = ldo(s.x, o), py = ldo(s.y, o), vx, vy;
+        float px = nx, py = ny, vx, vy;
+        if (j + kThreads < item.end) {
+            nx = ldo(s.x, o + 4u * kThreads);
+            ny = ldo(s.y, o + 4u * kThreads);
+        }
         const int mat = ldo(s.mat, (uint32_t)j);
         Mat2 C, B, F = ld_F(s, o);
         float J = ldo(s.J, o);
