@@ -28,9 +28,8 @@ typedef __attribute__((__vector_size__(16 * sizeof(float)))) float floatx16;
 
 // The packed network computes every hidden pre-activation in the base-2 scale s = -log2(e) z; then
 // silu(z) = z / (1 + e^-z) = -ln 2 * s / (1 + 2^s).  The factor -ln 2 is folded into the next layer:
-// W2 (-ln 2)(-log2 e) = W2 and W3 (-ln 2) (see k_pack_closure), so the activation is two VALU operations
-// and the hardware exp2 and reciprocal.
-__device__ __forceinline__ float silu_base2(float s) { return s * __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(s)); }
+// W2 (-ln 2)(-log2 e) = W2 and W3 (-ln 2) (see k_pack_closure), so the activation s * rcp(1 + exp2(s)) is
+// two VALU operations and the hardware exp2 and reciprocal (chain_operands).
 
 // atan2(y, x): atan on [0, 1] from the degree-9 odd polynomial of Abramowitz & Stegun 4.4.47 (|error| <=
 // 1e-5 rad) plus octant reduction.  Training and inference use this same function, so the network's angle
