@@ -647,6 +647,7 @@ int32_t mpm_params_preset(int32_t preset, MpmParams* out)
         p.num_scenes = 64;
         p.capacity = 64 * 524288;
         p.sort_interval = 100;
+        p.chunk = 3072;  // steady-state sweep: 2304..4096 all ~1.5 % faster than 2048
         break;
     case MPM_PRESET_LEARNED1024:
         unit_square(1024, 2e-5f, 16.f);
@@ -1027,7 +1028,7 @@ int32_t mpm_autotune(const MpmParams* base, int32_t n, const float* x, const flo
     if (best.sort_interval <= 0 || best.sort_interval > substeps) { best.sort_interval = substeps; }
     float best_ms = 3.4e38f;
     int32_t tried = 0;
-    const int chunks[4] = {256, 512, 1024, 2048};
+    const int chunks[4] = {512, 1024, 2048, 4096};
     for (int k = 0; k < kNumShapes; ++k) {
         for (int c : chunks) {
             MpmParams p = best;

@@ -382,7 +382,7 @@ def test_autotune_picks_a_valid_configuration(oracle_mod):
     sc = scenes.mixed(n=65536)
     tuned, res = autotune(p, sc)
     assert (res.block_x, res.block_y) in [(8, 8), (16, 8), (16, 16), (32, 16)]
-    assert res.chunk in (256, 512, 1024, 2048) and res.candidates >= 16
+    assert res.chunk in (512, 1024, 2048, 4096) and res.candidates >= 16
     assert res.ms_per_substep > 0
     out = run_gpu(tuned, sc.x, sc.v, sc.material)
     s, _ = run_oracle(oracle_mod, tuned, sc.x, sc.v, sc.material, tuned.substeps)
