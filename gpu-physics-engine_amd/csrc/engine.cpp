@@ -664,6 +664,7 @@ int32_t mpm_params_preset(int32_t preset, MpmParams* out)
         p.num_scenes = 16;
         p.capacity = 16 * 16777216 / 2;
         p.sort_interval = 100;
+        p.chunk = 3072;  // steady-state sweep: 417.9 ms per tick at 3072 and 4096 against 429.9 at 2048
         p.closure = MPM_CLOSURE_LEARNED;
         break;
     default:
