@@ -132,6 +132,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--scale", type=float, default=1.0, help="particle count scale (1.0 = 4,194,304)")
     ap.add_argument("--autotune", action="store_true", help="pick tile shape, chunk and sort interval by mpm_autotune")
+    ap.add_argument("--sort-interval", type=int, default=0, metavar="SUBSTEPS",
+                    help="substeps between binnings (0 = the preset's; above the tick's substeps: every k-th tick)")
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="mixed1024",
                     help="mixed1024 is the headline; the others are the remaining configurations")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -147,6 +149,8 @@ def main():
     preset, make_scene, description = WORKLOADS[args.workload]
     p = abi.preset(preset)
     p.mode = {"reference": abi.MODE_REFERENCE, "tiled": abi.MODE_TILED, "fused": abi.MODE_FUSED}[args.mode]
+    if args.sort_interval > 0:
+        p.sort_interval = args.sort_interval
     scene = make_scene(args.scale)
     p.capacity = scene.n
     log(f"scene: {scene.description}")

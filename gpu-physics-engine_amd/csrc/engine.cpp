@@ -30,10 +30,15 @@ struct MpmEngine {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t t0 = nullptr, t1 = nullptr, done = nullptr;
-    hipGraph_t graph[2] = {nullptr, nullptr};
-    hipGraphExec_t exec[2] = {nullptr, nullptr};
-    int flips = 0;  // store swaps per tick (parity)
-    int cur = 0;    // store holding the current state
+    // Tick graphs by starting store c and by whether the tick begins with a binning ([c][1]) or reuses the
+    // bins of an earlier tick ([c][0], built only when sort_interval spans several ticks).
+    hipGraph_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    hipGraphExec_t exec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    int flips[2] = {0, 0};      // store swaps per tick (parity) of each variant
+    int cur = 0;                // store holding the current state
+    int ticks_per_sort = 1;     // ceil(sort_interval / substeps) in the binned modes
+    int ticks_since_sort = 0;   // ticks run since the last tick that began with a binning
+    bool need_sort = true;      // the population or its order changed outside a tick: the next tick bins
     uint8_t* slab = nullptr;
     size_t slab_bytes = 0;
     Store store[2]{};
@@ -331,7 +336,8 @@ Grid grid_for(const MpmEngine* h, int target, int src, int zero)
 // Enqueue `substeps` substeps on the stream starting from store c (c is updated to the store holding the
 // result), then the output copy when `output` is set.  dirty: which accumulator sets hold data, before (in,
 // all zero) and after (out); every segment checks it scatters only into zero sets.
-hipError_t enqueue_substeps(MpmEngine* h, int& c, int substeps, bool output, LaunchTimer* tm, bool dirty[3])
+hipError_t enqueue_substeps(MpmEngine* h, int& c, int substeps, bool output, LaunchTimer* tm, bool dirty[3],
+                            bool bin_first = true)
 {
     const MpmParams& p = h->params;
     const DevParams& d = h->dp;
@@ -357,8 +363,10 @@ hipError_t enqueue_substeps(MpmEngine* h, int& c, int substeps, bool output, Lau
         int done = 0;
         while (done < substeps) {
             const int L = std::min(p.sort_interval, substeps - done);
-            ENQ(MPM_KT_BINNING, launch_binning(st, d, h->store[c], h->store[c ^ 1], h->count, h->ss));
-            c ^= 1;
+            if (bin_first || done > 0) {
+                ENQ(MPM_KT_BINNING, launch_binning(st, d, h->store[c], h->store[c ^ 1], h->count, h->ss));
+                c ^= 1;
+            }
             const Store& s = h->store[c];
             if (p.mode == MPM_MODE_TILED) {
                 for (int k = 0; k < L; ++k) {
@@ -403,24 +411,33 @@ int32_t build_graphs(MpmEngine* h)
 {
     const MpmParams& p = h->params;
     int segs = 0;
-    if (p.mode != MPM_MODE_REFERENCE) { segs = (p.substeps + p.sort_interval - 1) / p.sort_interval; }
-    h->flips = segs & 1;
-    for (int c = 0; c < 2; ++c) {
-        HIP_TRY(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-        int cc = c;
-        bool dirty[3] = {false, false, false};
-        hipError_t e = enqueue_substeps(h, cc, p.substeps, true, nullptr, dirty);
-        hipGraph_t g = nullptr;
-        hipError_t e2 = hipStreamEndCapture(h->stream, &g);
-        if (e != hipSuccess) {
-            if (g) { (void)hipGraphDestroy(g); }
-            return fail(h, MPM_ERR_DEVICE, (int32_t)e, std::string("tick capture: ") + hipGetErrorName(e));
+    h->ticks_per_sort = 1;
+    if (p.mode != MPM_MODE_REFERENCE) {
+        const int iv = std::min(p.sort_interval, p.substeps);
+        segs = (p.substeps + iv - 1) / iv;
+        h->ticks_per_sort = (p.sort_interval + p.substeps - 1) / p.substeps;
+    }
+    h->flips[1] = segs & 1;
+    h->flips[0] = (segs - 1) & 1;
+    for (int b = 1; b >= 0; --b) {
+        if (b == 0 && h->ticks_per_sort <= 1) { break; }
+        for (int c = 0; c < 2; ++c) {
+            HIP_TRY(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+            int cc = c;
+            bool dirty[3] = {false, false, false};
+            hipError_t e = enqueue_substeps(h, cc, p.substeps, true, nullptr, dirty, b == 1);
+            hipGraph_t g = nullptr;
+            hipError_t e2 = hipStreamEndCapture(h->stream, &g);
+            if (e != hipSuccess) {
+                if (g) { (void)hipGraphDestroy(g); }
+                return fail(h, MPM_ERR_DEVICE, (int32_t)e, std::string("tick capture: ") + hipGetErrorName(e));
+            }
+            if (e2 != hipSuccess) { return fail(h, MPM_ERR_DEVICE, (int32_t)e2, "hipStreamEndCapture failed"); }
+            h->graph[c][b] = g;
+            HIP_TRY(h, hipGraphInstantiate(&h->exec[c][b], g, nullptr, nullptr, 0));
+            // upload now: the first launch of an exec that is not uploaded leaves gaps between its kernels
+            HIP_TRY(h, hipGraphUpload(h->exec[c][b], h->stream));
         }
-        if (e2 != hipSuccess) { return fail(h, MPM_ERR_DEVICE, (int32_t)e2, "hipStreamEndCapture failed"); }
-        h->graph[c] = g;
-        HIP_TRY(h, hipGraphInstantiate(&h->exec[c], g, nullptr, nullptr, 0));
-        // upload now: the first launch of an exec that is not uploaded leaves gaps between its kernels
-        HIP_TRY(h, hipGraphUpload(h->exec[c], h->stream));
     }
     return MPM_OK;
 }
@@ -430,8 +447,10 @@ void release(MpmEngine* h)
     if (!h) { return; }
     if (h->stream) { (void)hipStreamSynchronize(h->stream); }
     for (int c = 0; c < 2; ++c) {
-        if (h->exec[c]) { (void)hipGraphExecDestroy(h->exec[c]); }
-        if (h->graph[c]) { (void)hipGraphDestroy(h->graph[c]); }
+        for (int b = 0; b < 2; ++b) {
+            if (h->exec[c][b]) { (void)hipGraphExecDestroy(h->exec[c][b]); }
+            if (h->graph[c][b]) { (void)hipGraphDestroy(h->graph[c][b]); }
+        }
     }
     if (h->t0) { (void)hipEventDestroy(h->t0); }
     if (h->t1) { (void)hipEventDestroy(h->t1); }
@@ -493,6 +512,7 @@ int32_t set_particles_impl(MpmEngine* h, int32_t n, const float* x, const float*
     const int32_t cnt[2] = {n, n};
     HIP_TRY(h, hipMemcpy(h->count, cnt, sizeof(cnt), hipMemcpyHostToDevice));
     h->last_count = n;
+    h->need_sort = true;
     return MPM_OK;
 }
 
@@ -638,7 +658,7 @@ int32_t mpm_params_preset(int32_t preset, MpmParams* out)
         lame(1e4f, 0.2f);
         p.substeps = 100;
         p.capacity = 4194304;
-        p.sort_interval = 100;
+        p.sort_interval = 200;  // every second tick: 7.68-7.76 ms per tick against 7.84 binning every tick
         break;
     case MPM_PRESET_BATCH64X256:
         unit_square(256, 2e-5f, 16.f);
@@ -758,10 +778,20 @@ int32_t mpm_step(MpmEngine* h, int32_t n_ticks)
         const int32_t slot = h->write_slot;
         HIP_TRY(h, set_output_view_async(h->stream, &h->staged[slot]));
         const bool last = k == n_ticks - 1;
+        // bin at the start of this tick unless the bins of an earlier tick are recent enough and nothing
+        // changed the population since
+        const int bin = (h->ticks_per_sort <= 1 || h->need_sort || h->ticks_since_sort + 1 >= h->ticks_per_sort) ? 1
+                                                                                                                 : 0;
         if (last) { HIP_TRY(h, hipEventRecord(h->t0, h->stream)); }
-        HIP_TRY(h, hipGraphLaunch(h->exec[h->cur], h->stream));
+        HIP_TRY(h, hipGraphLaunch(h->exec[h->cur][bin], h->stream));
         if (last) { HIP_TRY(h, hipEventRecord(h->t1, h->stream)); }
-        h->cur ^= h->flips;
+        h->cur ^= h->flips[bin];
+        if (bin) {
+            h->ticks_since_sort = 0;
+            h->need_sort = false;
+        } else {
+            ++h->ticks_since_sort;
+        }
         h->slot_count[slot].store(h->last_count, std::memory_order_relaxed);
         h->pending_slot = slot;
         h->write_slot = (slot + 1) % 3;
@@ -983,6 +1013,7 @@ int32_t mpm_hose_pipe(MpmEngine* h, float origin_x, float origin_y, float veloci
     if (e != hipSuccess) { return fail(h, MPM_ERR_DEVICE, (int32_t)e, "emitter launch"); }
     // the device keeps the first capacity - count particles (k_emitter), so the host count stays exact
     h->last_count = (int32_t)std::min<int64_t>(h->params.capacity, (int64_t)h->last_count + count);
+    h->need_sort = true;  // the new particles are in no bin yet
     if (emitted) { *emitted = count; }
     return MPM_OK;
 }
@@ -1123,6 +1154,8 @@ int32_t mpm_time_kernels(MpmEngine* h, int32_t substeps, float* ms, int32_t* lau
     if (e == hipSuccess) { e = tm.close(); }
     if (e != hipSuccess) { return fail(h, MPM_ERR_DEVICE, (int32_t)e, "timed launch"); }
     h->cur = c;
+    h->ticks_since_sort = 0;
+    h->need_sort = false;
     HIP_TRY(h, hipStreamSynchronize(h->stream));
     for (int k = 0; k < MPM_KT_COUNT; ++k) {
         ms[k] = 0.f;

@@ -809,6 +809,18 @@ struct PlaneTiles {
         return v;
     }
 };
+// Minimum waves per SIMD requested from the register allocator for the binned kernels of one tile shape:
+// the target (7 for the plain fused kernel, 5 for the learned kernels), capped by what the work group's LDS
+// allows on a CU (160 KB; one work group of 4 waves holds one wave per SIMD).
+template <int BX, int BY, bool kLearned, int kTarget>
+struct WavesPerSimd {
+    static constexpr int N = TileShape<BX, BY>::N;
+    static constexpr int lds = (int)(sizeof(PlaneTiles<N>) + sizeof(float2) * N) +
+                               (kLearned ? (int)sizeof(ClosurePacked) : 0);
+    static constexpr int by_lds = (160 * 1024) / lds;
+    static constexpr int value = by_lds < kTarget ? (by_lds < 1 ? 1 : by_lds) : kTarget;
+};
+
 template <int BX, int BY, class Tiles>
 __device__ __forceinline__ void zero_acc_tiles(Tiles& acc)
 {
@@ -1106,7 +1118,8 @@ __global__ __launch_bounds__(kThreads) void k_g2p_tiled(DevParams p, Store s, Gr
 template <int BX, int BY>
 // At least 7 waves per SIMD (at most 72 VGPRs): more resident waves hide the latency of the scatter's
 // dependent LDS read-add-write chain (8 would spill to scratch).
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7))) void k_g2p2g(
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WavesPerSimd<BX, BY, false, 7>::value))) void
+k_g2p2g(
     DevParams p, Store s, Grid g, const Item* items, const int32_t* num_items, const int32_t* count)
 {
     using T = TileShape<BX, BY>;
@@ -1178,7 +1191,8 @@ __device__ __forceinline__ Mat2 affine_q_learned(const DevParams& p, const Closu
 
 // k_p2g_tiled with the learned closure (uniform trip count so every wave runs the network with 64 lanes).
 template <int BX, int BY>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5))) void k_p2g_tiled_learned(DevParams p, Store s, Grid g, const Item* items,
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WavesPerSimd<BX, BY, true, 5>::value))) void
+k_p2g_tiled_learned(DevParams p, Store s, Grid g, const Item* items,
                                                                 const int32_t* num_items, const int32_t* count)
 {
     using T = TileShape<BX, BY>;
@@ -1213,7 +1227,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5))) v
 // k_g2p2g with the learned closure: the P2G stress of every wave comes from one BF16 MFMA evaluation of
 // the network for its 64 particles.
 template <int BX, int BY>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5))) void k_g2p2g_learned(DevParams p, Store s, Grid g, const Item* items,
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WavesPerSimd<BX, BY, true, 5>::value))) void
+k_g2p2g_learned(DevParams p, Store s, Grid g, const Item* items,
                                                             const int32_t* num_items, const int32_t* count)
 {
     using T = TileShape<BX, BY>;

@@ -89,7 +89,9 @@ typedef struct MpmParams {
     int32_t substeps;  /* substeps per mpm_step tick (kSubsteps, constants.h:72)                              */
     int32_t capacity;  /* particle capacity over all scenes (kNumAtoms, constants.h:83)                       */
     int32_t num_scenes;/* independent scenes, each with its own grid_w x grid_h grid (batched form)           */
-    int32_t sort_interval; /* substeps between re-binnings inside a tick (>=1; binned modes only)            */
+    int32_t sort_interval; /* substeps between re-binnings (>=1; binned modes only).  Up to `substeps`: within
+                              a tick.  Above: a tick starts with a binning every ceil(sort_interval /
+                              substeps) ticks, and always after the particles were set, loaded or emitted    */
     int32_t mode;      /* MPM_MODE_*                                                                          */
     int32_t closure;   /* MPM_CLOSURE_*                                                                       */
     int32_t block_x, block_y; /* cells per bin along x / y (0 = tuned default)                               */
