@@ -171,11 +171,13 @@ def main():
         eng.step(1)
     eng.sync()
     log(f"warmup {args.warmup} ticks done")
-    # Per-kernel HIP-event timing on the engine's stream: one eagerly launched tick right before and one
-    # right after the timed region (events cannot sit inside the tick graph).  The scene compresses under
-    # gravity and the fused kernel slows as it does, so the two ticks bracket the timed region and their
-    # sum stands for its average.
-    kt_before = eng.time_kernels(p.substeps)
+    # Per-kernel HIP-event timing on the engine's stream: two binning cycles of eagerly launched ticks (a
+    # cycle is one tick, or every tick up to the next binning when sort_interval spans several) right
+    # before and two right after the timed region (events cannot sit inside the tick graph).  The scene
+    # compresses under gravity and the fused kernel slows as it does, so they bracket the timed region and
+    # their sum stands for its average (single ticks vary by a few per cent).
+    cycle = 2 * p.substeps * (max(1, -(-p.sort_interval // p.substeps)) if p.mode != abi.MODE_REFERENCE else 1)
+    kt_before = eng.time_kernels(cycle)
     t0 = time.perf_counter()
     eng.step(args.steps)
     eng.sync()
@@ -184,7 +186,7 @@ def main():
     ms_per_step = wall * 1e3 / args.steps
     value = scene.n * p.substeps * args.steps / wall
     log(f"timed {args.steps} ticks: {ms_per_step:.3f} ms/tick (last tick by HIP events {last_tick_ms:.3f} ms)")
-    kt_after = eng.time_kernels(p.substeps)
+    kt_after = eng.time_kernels(cycle)
     kt = {k: (kt_before[k][0] + kt_after[k][0], kt_before[k][1] + kt_after[k][1]) for k in kt_before}
     kernels = {k: {"ms_total": round(ms, 4), "launches": n, "us_avg": round(1e3 * ms / n, 2) if n else None}
                for k, (ms, n) in kt.items()}

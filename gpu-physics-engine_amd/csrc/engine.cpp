@@ -407,6 +407,22 @@ hipError_t enqueue_substeps(MpmEngine* h, int& c, int substeps, bool output, Lau
     return hipSuccess;
 }
 
+// Whether the next tick starts with a binning: always when sort_interval fits in a tick, otherwise every
+// ticks_per_sort ticks and whenever the population changed outside a tick (set, load, emission).
+bool next_tick_bins(const MpmEngine* h)
+{
+    return h->ticks_per_sort <= 1 || h->need_sort || h->ticks_since_sort + 1 >= h->ticks_per_sort;
+}
+void note_tick(MpmEngine* h, bool binned)
+{
+    if (binned) {
+        h->ticks_since_sort = 0;
+        h->need_sort = false;
+    } else {
+        ++h->ticks_since_sort;
+    }
+}
+
 int32_t build_graphs(MpmEngine* h)
 {
     const MpmParams& p = h->params;
@@ -778,20 +794,12 @@ int32_t mpm_step(MpmEngine* h, int32_t n_ticks)
         const int32_t slot = h->write_slot;
         HIP_TRY(h, set_output_view_async(h->stream, &h->staged[slot]));
         const bool last = k == n_ticks - 1;
-        // bin at the start of this tick unless the bins of an earlier tick are recent enough and nothing
-        // changed the population since
-        const int bin = (h->ticks_per_sort <= 1 || h->need_sort || h->ticks_since_sort + 1 >= h->ticks_per_sort) ? 1
-                                                                                                                 : 0;
+        const int bin = next_tick_bins(h) ? 1 : 0;
         if (last) { HIP_TRY(h, hipEventRecord(h->t0, h->stream)); }
         HIP_TRY(h, hipGraphLaunch(h->exec[h->cur][bin], h->stream));
         if (last) { HIP_TRY(h, hipEventRecord(h->t1, h->stream)); }
         h->cur ^= h->flips[bin];
-        if (bin) {
-            h->ticks_since_sort = 0;
-            h->need_sort = false;
-        } else {
-            ++h->ticks_since_sort;
-        }
+        note_tick(h, bin);
         h->slot_count[slot].store(h->last_count, std::memory_order_relaxed);
         h->pending_slot = slot;
         h->write_slot = (slot + 1) % 3;
@@ -1150,12 +1158,18 @@ int32_t mpm_time_kernels(MpmEngine* h, int32_t substeps, float* ms, int32_t* lau
     tm.st = h->stream;
     int c = h->cur;
     bool dirty[3] = {false, false, false};
-    hipError_t e = enqueue_substeps(h, c, substeps, false, &tm, dirty);
+    // whole ticks follow the binning schedule of mpm_step; a remainder shorter than a tick bins first
+    hipError_t e = hipSuccess;
+    for (int left = substeps; e == hipSuccess && left > 0;) {
+        const int n = std::min(left, h->params.substeps);
+        const bool bin = n < h->params.substeps || next_tick_bins(h);
+        e = enqueue_substeps(h, c, n, false, &tm, dirty, bin);
+        note_tick(h, bin);
+        left -= n;
+    }
     if (e == hipSuccess) { e = tm.close(); }
     if (e != hipSuccess) { return fail(h, MPM_ERR_DEVICE, (int32_t)e, "timed launch"); }
     h->cur = c;
-    h->ticks_since_sort = 0;
-    h->need_sort = false;
     HIP_TRY(h, hipStreamSynchronize(h->stream));
     for (int k = 0; k < MPM_KT_COUNT; ++k) {
         ms[k] = 0.f;

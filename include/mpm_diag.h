@@ -37,7 +37,9 @@ int32_t mpm_debug_bin(MpmEngine* h, uint32_t* keys, int32_t* ids, int32_t* n);
  * bracketing every run of consecutive launches of one class (for example the fused substeps of a
  * segment, which run back to back) with a pair of HIP events on the engine stream.  ms[k] receives the
  * summed GPU time of class k, launches[k] the number of launches (arrays of MPM_KT_COUNT).  The output
- * slot is not written.  Advances the simulation exactly like mpm_step would. */
+ * slot is not written.  Advances the simulation exactly like mpm_step would: whole ticks follow mpm_step's
+ * binning schedule (when sort_interval spans several ticks, only some ticks start with a binning); a
+ * remainder shorter than a tick starts with one. */
 int32_t mpm_time_kernels(MpmEngine* h, int32_t substeps, float* ms, int32_t* launches);
 
 /* Kirchhoff stress of n particles with the handle's material parameters, computed by the engine's

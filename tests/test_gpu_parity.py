@@ -404,6 +404,11 @@ def test_binning_every_second_tick(oracle_mod):
         p.capacity = sc.n
         outs[iv] = run_gpu(p, sc.x, sc.v, sc.material, ticks=4)
     s, _ = run_oracle(oracle_mod, p, sc.x, sc.v, sc.material, 100)
+    with Engine(p) as e:  # the eager timer follows the same schedule: one binning in two ticks
+        e.set_scene(sc)
+        kt = e.time_kernels(2 * p.substeps)
+        assert kt["binning"][1] == 1 and kt["p2g"][1] == 2 and kt["g2p"][1] == 2
+        assert kt["g2p2g"][1] == 2 * (p.substeps - 1)
     assert rel(outs[50]["x"], outs[25]["x"]) < 1e-6
     assert rel(outs[50]["v"], outs[25]["v"]) < 1e-3
     assert rel(outs[50]["F"], outs[25]["F"]) < 1e-5
