@@ -391,8 +391,8 @@ def test_autotune_picks_a_valid_configuration(oracle_mod):
 
 def test_binning_every_second_tick(oracle_mod):
     """sort_interval above the tick's substeps: ticks alternate between binning first and reusing the bins of
-    the tick before (graph variants on both stores).  Four ticks of the scaled mixed scene agree with
-    binning every tick and with the FP64 oracle; and with hose-pipe emission before every tick (each
+    the tick before (graph variants on both stores).  Four ticks of the scaled mixed scene (fused, and tiled)
+    agree with binning every tick and with the FP64 oracle; and with hose-pipe emission before every tick (each
     emission forces a binning, since the new particles are in no bin) the reference configuration agrees
     with binning every tick."""
     sc = scenes.mixed(n=131072)
@@ -403,6 +403,10 @@ def test_binning_every_second_tick(oracle_mod):
         p.sort_interval = iv
         p.capacity = sc.n
         outs[iv] = run_gpu(p, sc.x, sc.v, sc.material, ticks=4)
+    pt = p.copy()
+    pt.mode = abi.MODE_TILED
+    tiled = run_gpu(pt, sc.x, sc.v, sc.material, ticks=4)
+    assert rel(tiled["x"], outs[25]["x"]) < 1e-6
     s, _ = run_oracle(oracle_mod, p, sc.x, sc.v, sc.material, 100)
     with Engine(p) as e:  # the eager timer follows the same schedule: one binning in two ticks
         e.set_scene(sc)
