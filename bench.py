@@ -36,6 +36,9 @@ G2P2G_BYTES_PER_NODE = 36
 # Whole-substep algorithmic bytes of the unfused three-kernel form: 132 B per particle, 40 B per node.
 SUBSTEP_BYTES_PER_PARTICLE = 132
 SUBSTEP_BYTES_PER_NODE = 40
+# Algorithmic bytes per particle of one binning: keys and ids written 8, three 8-bit radix passes reading and
+# writing (key, id) 3 x 16, the layered order 12, the full-state gather reading and writing 57 B each.
+BINNING_BYTES_PER_PARTICLE = 8 + 3 * 16 + 12 + 2 * 57
 
 
 WORKLOADS = {
@@ -207,6 +210,17 @@ def main():
     achieved = bytes_per_launch / (ms_dom / n_dom * 1e-3) / 1e9 if n_dom else 0.0
     substep_s = wall / (args.steps * p.substeps)
     substep_bytes = scene.n * SUBSTEP_BYTES_PER_PARTICLE + nodes * SUBSTEP_BYTES_PER_NODE
+    # the binning on its own line, amortised over the substeps that the event-timed cycles covered
+    ms_bin, n_bin = kt["binning"]
+    binning = None
+    if n_bin:
+        us_sort = 1e3 * ms_bin / n_bin
+        bin_bytes = scene.n * BINNING_BYTES_PER_PARTICLE
+        binning = {"us_per_sort": us_sort, "bytes_per_sort": bin_bytes,
+                   "achieved": bin_bytes / (us_sort * 1e-6) / 1e9, "unit": "GB/s",
+                   "frac": bin_bytes / (us_sort * 1e-6) / 1e9 / HBM_PEAK_GBS,
+                   "substeps_per_sort": 2 * cycle / n_bin,
+                   "us_per_substep_amortised": 1e3 * ms_bin / (2 * cycle)}
 
     result = {
         "metric": METRIC,
@@ -234,6 +248,7 @@ def main():
                      "us_per_launch": 1e3 * ms_dom / n_dom if n_dom else None},
         "substep": {"us": substep_s * 1e6, "algorithmic_bytes_unfused": substep_bytes,
                     "effective_GBs": substep_bytes / substep_s / 1e9},
+        "binning": binning,
         "kernels": kernels,
         "state_finite": finite,
     }

@@ -266,6 +266,35 @@ def test_stress_kernel_against_oracle(oracle_mod):
     assert np.max(np.abs(tau - ref3) / scale) < 1e-4
 
 
+def test_tait_pressure_within_ulp_bound(oracle_mod):
+    """Tait pressure with gamma = 3.2 (powf, MPM.cu:68-69) against FP64 std::pow, per particle, within the
+    FP32 error budget of its steps: 1/J rounded (relative 2^-24, amplified gamma times by the power), powf
+    (at most 2 ulp), k(.. - 1) and -J p rounded (2^-24 each, taken as 2^-22 of |tau|).  J covers both clamps
+    (J > 1 gives p_hat < 0 -> p_min = -0; J = 0.1 gives p_hat ~ 1.3e8 -> p_max) and J = 1 exactly."""
+    p = abi.preset(abi.PRESET_REFERENCE)
+    rng = np.random.default_rng(33)
+    J = np.concatenate([rng.uniform(0.55, 1.45, 3000), [1.0, 0.1, 0.25, 0.5, 2.0, 10.0]]).astype(np.float32)
+    n = J.size
+    F = np.tile(np.eye(2, dtype=np.float32).reshape(1, 2, 2), (n, 1, 1))
+    mat = np.zeros(n, np.uint8)  # fluid
+    with Engine(p) as e:
+        tau = e.compute_stress(mat, F, J)
+    J64 = J.astype(np.float64)
+    ref = oracle_mod.tau(p, mat, F.astype(np.float64), J64)
+    assert np.all(ref[:, 1] == 0.0) and np.all(tau[:, 1] == 0.0)
+    pw = (1.0 / J64) ** np.float64(p.tait_gamma)
+    gamma, k = float(np.float32(p.tait_gamma)), float(np.float32(p.tait_k))
+    unclamped = (k * (pw - 1.0) > float(p.p_min)) & (k * (pw - 1.0) < float(p.p_max))
+    bound = np.where(unclamped, J64 * k * pw * (gamma + 4.0) * 2.0**-24, 0.0) + 2.0**-22 * np.abs(ref[:, 0])
+    err = np.abs(tau[:, 0].astype(np.float64) - ref[:, 0])
+    assert np.all(err <= bound), (J[np.argmax(err - bound)], np.max(err - bound))
+    assert np.array_equal(tau[:, 0], tau[:, 2])
+    # the clamps: J > 1 has zero pressure, J = 0.1 sits at p_max = 8.2e5 (tau = -J p_max)
+    assert np.all(tau[J > 1.0, 0] == 0.0)
+    assert tau[n - 6 + 1, 0] == np.float32(-0.1) * np.float32(p.p_max)
+    assert tau[n - 6, 0] == 0.0  # J = 1: (1/J)^gamma - 1 = 0 exactly
+
+
 def test_output_view_and_emitter():
     """Hose-pipe emission appends width*rows particles (MPM.cu:450-453) and the output slot written by a
     tick holds the state in caller order (MPMOutputKernel, MPM.cu:302-322)."""
