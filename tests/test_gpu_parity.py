@@ -119,6 +119,54 @@ def test_crowded_start_nodes(oracle_mod, mode):
 
 
 @pytest.mark.parametrize("mode", MODES, ids=lambda m: MODE_NAMES[m])
+@pytest.mark.parametrize("substeps", [1, 4])
+def test_forced_failures_against_oracle(oracle_mod, mode, substeps):
+    """Failure handling forced from the host, as the oracle does it:
+    - a NaN and an Inf velocity component (handleOOB, MPM.cu:277-284): the non-finite momentum reaches
+      the particle's nodes and every particle gathering from them; each affected axis is reset to the
+      domain centre with v = 0, and F turns non-finite;
+    - particles at the left wall moving out fast (MPM.cu:285-300): x clamps to 0 and v_x to max(v_x, 0);
+    - J outside [0.1, 10] (MPM.cu:265): clamped after the first G2P.
+    The set of particles with a non-finite F must equal the oracle's exactly."""
+    rng = np.random.default_rng(77)
+    n = 1500
+    x = rng.uniform(0.3, 0.7, (n, 2))
+    v = rng.uniform(-0.5, 0.5, (n, 2))
+    mat = rng.integers(0, 2, n).astype(np.uint8)  # fluid and jelly
+    J = np.ones(n)
+    v[10] = [np.nan, 0.0]
+    v[20] = [0.0, np.inf]
+    # -200 leaves the wall within one substep and keeps the run well conditioned (a 1e-7 relative change of
+    # v moves the FP64 result by 5e-12); at -2000 the particles pile into a corner at the J clamp, where the
+    # same change moves it by 8e-4 and no FP32 run can follow the oracle
+    x[30:36, 0] = 0.002
+    v[30:36] = [-200.0, 0.0]
+    J[40], J[41] = 0.05, 20.0
+    mat[40:42] = abi.MAT_FLUID
+    x, v, J = x.astype(np.float32), v.astype(np.float32), J.astype(np.float32)
+    p = small_params(substeps=substeps, mode=mode, rho=4.0)
+    out = run_gpu(p, x, v, mat, J=J)
+    s, grid = run_oracle(oracle_mod, p, x, v, mat, substeps, threads=1, J=J)
+    bad_gpu = ~np.isfinite(out["F"]).all(axis=1)
+    bad_ref = ~np.isfinite(s.F).all(axis=1)
+    assert bad_ref.sum() > 50 and np.array_equal(bad_gpu, bad_ref)
+    assert np.isfinite(out["x"]).all() and np.isfinite(out["v"]).all()
+    # every reset axis sits exactly at the centre, as in the oracle
+    assert np.array_equal(out["x"] == 0.5, s.x == 0.5)
+    assert rel(out["x"], s.x) < 1e-6
+    assert rel(out["v"], s.v) < 1e-4
+    assert np.all(out["x"][30:36, 0] >= 0.0) and np.all(out["x"][30:36, 0] < 1e-6)
+    assert out["J"][41] == np.float32(p.j_max)
+    assert np.isfinite(out["J"]).all() and out["J"].min() >= np.float32(p.j_min)
+    good = ~bad_ref
+    assert rel(out["J"][good], s.J[good]) < 1e-6
+    if substeps == 1:
+        assert np.all(out["x"][30:36, 0] == 0.0) and np.all(out["v"][30:36, 0] == 0.0)
+        assert out["J"][40] == np.float32(p.j_min)
+    assert rel(out["grid"][0][0], grid[0]) < 1e-5
+
+
+@pytest.mark.parametrize("mode", MODES, ids=lambda m: MODE_NAMES[m])
 def test_jelly_block_100_substeps(oracle_mod, mode):
     """The 128-cell jelly block (32,768 particles) after 100 substeps against the FP64 oracle."""
     p = abi.preset(abi.PRESET_JELLY128)
