@@ -10,7 +10,8 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle_mpm.so")
+# MPM_ORACLE_LIB selects another build of the same source (the sanitized one of `make sanitized`).
+LIB_PATH = os.environ.get("MPM_ORACLE_LIB") or os.path.join(HERE, "liboracle_mpm.so")
 
 _lib = None
 _d = ctypes.POINTER(ctypes.c_double)
