@@ -36,9 +36,16 @@ G2P2G_BYTES_PER_NODE = 36
 # Whole-substep algorithmic bytes of the unfused three-kernel form: 132 B per particle, 40 B per node.
 SUBSTEP_BYTES_PER_PARTICLE = 132
 SUBSTEP_BYTES_PER_NODE = 40
-# Algorithmic bytes per particle of one binning: keys and ids written 8, three 8-bit radix passes reading and
-# writing (key, id) 3 x 16, the layered order 12, the full-state gather reading and writing 57 B each.
-BINNING_BYTES_PER_PARTICLE = 8 + 3 * 16 + 12 + 2 * 57
+
+
+def binning_bytes_per_particle(p):
+    """Algorithmic bytes per particle of one binning: keys from x, y (read 8, write 4); P radix passes of 8
+    bits over the bin bits (the histogram reads the key 4, the scatter reads and writes (key, id) 16); the
+    bin ranges read the key 4; the layered order reads the key twice and the id and writes both (20); the
+    full-state gather reads the id 4 and 57 B of state and writes 57 B."""
+    bins = p.num_scenes * (-(-p.grid_w // p.block_x)) * (-(-p.grid_h // p.block_y))
+    passes = -(-max(1, (bins - 1).bit_length()) // 8)
+    return 12 + 20 * passes + 4 + 20 + 4 + 2 * 57
 
 
 WORKLOADS = {
@@ -215,7 +222,7 @@ def main():
     binning = None
     if n_bin:
         us_sort = 1e3 * ms_bin / n_bin
-        bin_bytes = scene.n * BINNING_BYTES_PER_PARTICLE
+        bin_bytes = scene.n * binning_bytes_per_particle(p)
         binning = {"us_per_sort": us_sort, "bytes_per_sort": bin_bytes,
                    "achieved": bin_bytes / (us_sort * 1e-6) / 1e9, "unit": "GB/s",
                    "frac": bin_bytes / (us_sort * 1e-6) / 1e9 / HBM_PEAK_GBS,

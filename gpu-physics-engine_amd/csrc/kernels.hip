@@ -293,6 +293,8 @@ __global__ __launch_bounds__(kThreads) void k_bin_keys(DevParams p, Store s, con
 }
 
 // Radix upsweep: digit histogram of one tile of kRadixTile keys, written digit-major hist[d * ntiles + t].
+// The keys arrive nearly sorted (the previous binning's order), so the 64 keys a wave reads mostly share
+// one digit: such a wave adds its count with one LDS atomic instead of 64 on the same counter.
 __global__ __launch_bounds__(kThreads) void k_radix_upsweep(const uint32_t* keys, const int32_t* count, int shift,
                                                             int32_t* hist, int ntiles)
 {
@@ -306,7 +308,15 @@ __global__ __launch_bounds__(kThreads) void k_radix_upsweep(const uint32_t* keys
 #pragma unroll 4
         for (int r = 0; r < kRadixItems; ++r) {
             const int idx = base + r * kThreads + threadIdx.x;
-            if (idx < n) { atomicAdd(&lh[(keys[idx] >> shift) & (kRadixBins - 1)], 1); }
+            const bool valid = idx < n;
+            const int d = valid ? (int)((keys[idx] >> shift) & (kRadixBins - 1)) : 0;
+            const int d0 = __shfl(d, 0, 64);
+            const uint64_t live = __ballot(valid);
+            if (__ballot(valid && d == d0) == live) {
+                if ((threadIdx.x & 63) == 0 && live != 0ull) { atomicAdd(&lh[d0], __popcll(live)); }
+            } else if (valid) {
+                atomicAdd(&lh[d], 1);
+            }
         }
     }
     __syncthreads();
@@ -356,23 +366,14 @@ __global__ __launch_bounds__(kThreads) void k_scan_partials(const int32_t* data,
     block_exclusive_scan(s, &total);
     if (threadIdx.x == 0) { partials[blockIdx.x] = total; }
 }
-// Scan phase 2: exclusive scan of the segment totals by one work group (any count).
-__global__ __launch_bounds__(kThreads) void k_scan_single(int32_t* partials, int nseg)
-{
-    int carry = 0;
-    for (int base = 0; base < nseg; base += kThreads) {
-        const int i = base + threadIdx.x;
-        const int v = i < nseg ? partials[i] : 0;
-        int total;
-        const int pre = block_exclusive_scan(v, &total);
-        if (i < nseg) { partials[i] = carry + pre; }
-        carry += total;
-        __syncthreads();
-    }
-}
-// Scan phase 3: exclusive scan of each segment in place, offset by its segment prefix.
+// Scan phase 2: exclusive scan of each segment in place, offset by the sum of the totals of the segments
+// before it (each work group adds them up itself: there are few segments, and one launch less).
 __global__ __launch_bounds__(kThreads) void k_scan_apply(int32_t* data, int len, const int32_t* partials)
 {
+    int before = 0;
+    for (int b = threadIdx.x; b < (int)blockIdx.x; b += kThreads) { before += partials[b]; }
+    int prefix;
+    block_exclusive_scan(before, &prefix);
     const int base = blockIdx.x * kScanSeg + threadIdx.x * kScanItems;
     int vals[kScanItems];
     int s = 0;
@@ -383,7 +384,7 @@ __global__ __launch_bounds__(kThreads) void k_scan_apply(int32_t* data, int len,
         s += vals[k];
     }
     int total;
-    int run = block_exclusive_scan(s, &total) + partials[blockIdx.x];
+    int run = block_exclusive_scan(s, &total) + prefix;
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
         const int i = base + k;
@@ -397,7 +398,6 @@ static hipError_t exclusive_scan(hipStream_t st, int32_t* data, int len, int32_t
     const int nseg = (len + kScanSeg - 1) / kScanSeg;
     if (nseg == 0) { return hipSuccess; }
     hipLaunchKernelGGL(k_scan_partials, dim3(nseg), dim3(kThreads), 0, st, data, len, partials);
-    hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(kThreads), 0, st, partials, nseg);
     hipLaunchKernelGGL(k_scan_apply, dim3(nseg), dim3(kThreads), 0, st, data, len, partials);
     return hipGetLastError();
 }
@@ -511,14 +511,16 @@ __global__ __launch_bounds__(kThreads) void k_bin_bounds(const uint32_t* keys, c
     if (j == n - 1 || (keys[j + 1] >> shift) != k) { bend[k] = j + 1; }
 }
 
-// Layered order within each bin.  Input: particles sorted stably by (bin, stencil base node).  Particle j
-// of node n has rank r = j - (first index of n); it goes to layer min(r, kLayers - 1).  Layers follow one
-// another, and inside a layer below the last each node holds one particle, so the 64 lanes of a wave
+// Layered order within each bin.  Input: particles sorted stably by bin (the radix passes skip the node
+// bits of the key), so inside a bin they keep their index order.  Particle j of node n has rank r = the
+// number of particles of node n before it in the bin; it goes to layer min(r, kLayers - 1).  Layers follow
+// one another, and inside a layer below the last each node holds one particle, so the 64 lanes of a wave
 // mostly start their stencils at distinct nodes and the ranked scatter needs fewer rounds.  Inside a layer
 // the nodes follow in checkerboard order: first those with lx + ly even, then those with lx + ly odd, each
 // group in node order.  The lanes of a wave then start two cells (or one diagonal step) apart, so particles
 // that drift by a cell during the substeps between two sorts still mostly start at distinct nodes.  The
-// last layer takes the remaining particles in (checkerboard slot, rank) order.  One work group per bin.
+// last layer takes the remaining particles in (checkerboard slot, rank) order.  One work group per bin; the
+// result equals the layered order of a stable sort by (bin, node).
 constexpr int kLayers = 16;
 constexpr int kMaxBinNodes = 512;  // largest bin (32 x 16 cells)
 
@@ -538,7 +540,9 @@ __global__ __launch_bounds__(kThreads) void k_layer_bins(const uint32_t* keys_in
                                                          int node_bits, int bin_x, uint32_t* keys_out,
                                                          int32_t* perm_out)
 {
-    __shared__ int st[kMaxBinNodes];                 // first index of each node
+    constexpr int kWaves = kThreads / 64;
+    __shared__ int wrun[kWaves][kMaxBinNodes];       // per wave and node: particles before the wave's share,
+                                                     // then the running count while the wave ranks its share
     __shared__ int cnt[kMaxBinNodes];                // particles of each node
     __shared__ int pre[kLayers][kMaxBinNodes];       // layer < last: nodes before n in the layer; last: particles
     __shared__ int tot[kLayers][kMaxBinNodes / 64];  // per layer and node wave: nodes (or particles) it holds
@@ -550,19 +554,31 @@ __global__ __launch_bounds__(kThreads) void k_layer_bins(const uint32_t* keys_in
     const int nn = 1 << node_bits;
     const uint32_t mask = (uint32_t)nn - 1u;
     const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    // each wave ranks a contiguous share of the bin, in whole groups of 64 particles
+    const int len = e - s;
+    const int gper = (((len + 63) >> 6) + kWaves - 1) / kWaves;
+    const int ws = s + min(wave * gper * 64, len);
+    const int we = s + min((wave + 1) * gper * 64, len);
     for (int n = threadIdx.x; n < nn; n += kThreads) {
-        st[n] = 0;
-        cnt[n] = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) { wrun[w][n] = 0; }
     }
     __syncthreads();
-    for (int j = s + threadIdx.x; j < e; j += kThreads) {
-        const int n = (int)(keys_in[j] & mask);
-        if (j == s || (int)(keys_in[j - 1] & mask) != n) { st[n] = j; }
-        if (j == e - 1 || (int)(keys_in[j + 1] & mask) != n) { cnt[n] = j + 1; }  // one past the run, for now
-    }
+    for (int j = ws + lane; j < we; j += 64) { atomicAdd(&wrun[wave][keys_in[j] & mask], 1); }
     __syncthreads();
-    for (int n = threadIdx.x; n < nn; n += kThreads) { cnt[n] = cnt[n] ? cnt[n] - st[n] : 0; }
+    // per node: total, and the exclusive prefix over the waves' shares
+    for (int n = threadIdx.x; n < nn; n += kThreads) {
+        int c = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+            const int t = wrun[w][n];
+            wrun[w][n] = c;
+            c += t;
+        }
+        cnt[n] = c;
+    }
     __syncthreads();
     // in-wave prefixes over checkerboard slots by ballot (layers below the last) and by a wave scan (last)
     for (int q = threadIdx.x; q < nn; q += kThreads) {
@@ -605,13 +621,32 @@ __global__ __launch_bounds__(kThreads) void k_layer_bins(const uint32_t* keys_in
         }
     }
     __syncthreads();
-    for (int j = s + threadIdx.x; j < e; j += kThreads) {
-        const uint32_t key = keys_in[j];
+    // Each wave walks its share in index order, 64 particles at a time: the rank of a particle is the
+    // wave's running count of its node plus the lanes below it with the same node (an exact multisplit,
+    // one ballot per node bit); the highest lane of each node then advances the running count.  A wave
+    // executes its LDS instructions in order, so the next group reads the advanced counts.
+    for (int base = ws; base < we; base += 64) {
+        const int j = base + lane;
+        const bool valid = j < we;
+        const uint32_t key = valid ? keys_in[j] : 0u;
         const int n = (int)(key & mask);
-        const int r = j - st[n];
-        const int off = r < kLayers - 1 ? loff[r] + pre[r][n] : loff[kLayers - 1] + pre[kLayers - 1][n] + (r - (kLayers - 1));
-        perm_out[s + off] = perm_in[j];
-        keys_out[s + off] = key >> node_bits;
+        uint64_t peers = __ballot(valid);
+        for (int b = 0; b < node_bits; ++b) {
+            const bool bit = (n >> b) & 1;
+            const uint64_t m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        int r = 0;
+        if (valid) { r = wrun[wave][n] + __popcll(peers & lt_mask); }
+        __asm__ __volatile__("" ::: "memory");
+        if (valid && (peers >> lane) == 1ull) { wrun[wave][n] = r + 1; }
+        __asm__ __volatile__("" ::: "memory");
+        const int off = r < kLayers - 1 ? loff[r] + pre[r][n]
+                                        : loff[kLayers - 1] + pre[kLayers - 1][n] + (r - (kLayers - 1));
+        if (valid && off < len) {  // always true for a correct ranking; keeps a wrong one inside the bin
+            perm_out[s + off] = perm_in[j];
+            keys_out[s + off] = key >> node_bits;
+        }
     }
 }
 // Work items per bin: ceil(count / chunk).
@@ -650,14 +685,16 @@ hipError_t launch_binning(hipStream_t st, const DevParams& p, const Store& src, 
     const int cap = ss.capacity;
     const unsigned pblocks = (unsigned)((cap + kThreads - 1) / kThreads);
     hipLaunchKernelGGL(k_bin_keys, dim3(pblocks), dim3(kThreads), 0, st, p, src, count, ss.keys_tmp);
-    const int passes = (ss.key_bits + kRadixBits - 1) / kRadixBits;
-    if (passes == 0) { return hipErrorInvalidValue; }
+    // the passes sort by the bin bits only: k_layer_bins ranks the nodes inside each bin itself
+    const int bin_bits = ss.key_bits - p.node_bits;
+    const int passes = (bin_bits + kRadixBits - 1) / kRadixBits;
+    if (passes <= 0) { return hipErrorInvalidValue; }
     const uint32_t* kin = ss.keys_tmp;
     const int32_t* vin = nullptr;
     for (int pass = 0; pass < passes; ++pass) {
         uint32_t* kout = pass % 2 == 0 ? ss.keys_alt : ss.keys_tmp;
         int32_t* vout = pass % 2 == 0 ? ss.vals_alt : ss.vals_tmp;
-        const int shift = pass * kRadixBits;
+        const int shift = p.node_bits + pass * kRadixBits;
         hipLaunchKernelGGL(k_radix_upsweep, dim3(ss.ntiles), dim3(kThreads), 0, st, kin, count, shift, ss.hist,
                            ss.ntiles);
         hipError_t e = exclusive_scan(st, ss.hist, kRadixBins * ss.ntiles, ss.partials);
@@ -667,7 +704,7 @@ hipError_t launch_binning(hipStream_t st, const DevParams& p, const Store& src, 
         kin = kout;
         vin = vout;
     }
-    // kin / vin: keys sorted by (bin, node) and the permutation; bin ranges, then the layered order
+    // kin / vin: keys sorted stably by bin and the permutation; bin ranges, then the layered order
     hipError_t e = launch_fill_i32(st, ss.bin_start, (size_t)ss.num_keys, 0);
     if (e != hipSuccess) { return e; }
     e = launch_fill_i32(st, ss.bin_end, (size_t)ss.num_keys, 0);
