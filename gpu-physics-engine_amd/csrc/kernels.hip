@@ -522,7 +522,6 @@ __global__ __launch_bounds__(kThreads) void k_bin_bounds(const uint32_t* keys, c
 // last layer takes the remaining particles in (checkerboard slot, rank) order.  One work group per bin; the
 // result equals the layered order of a stable sort by (bin, node).
 constexpr int kLayers = 16;
-constexpr int kMaxBinNodes = 512;  // largest bin (32 x 16 cells)
 
 // Node (ly * bin_x + lx) at checkerboard slot q of a bin of nn = bin_x * bin_y nodes, bin_x even: slots
 // [0, nn/2) are the nodes with lx + ly even, slots [nn/2, nn) those with lx + ly odd, each in node order.
@@ -535,23 +534,39 @@ __device__ __forceinline__ int checker_node(int q, int nn, int bin_x)
     const int lx = 2 * (r - ly * hx) + ((ly + c) & 1);
     return ly * bin_x + lx;
 }
+// LDS of k_layer_bins for bins of 2^node_bits nodes (at least 64: the node prefixes run over whole waves);
+// 43.6 KB for the largest compiled shape (32 x 16 cells, 512 nodes).
+static size_t layer_bins_lds_bytes(int node_bits)
+{
+    const size_t nn = (size_t)1 << node_bits;
+    return sizeof(int) * ((size_t)(kThreads / 64) * nn + nn + (size_t)kLayers * nn + (size_t)kLayers * ((nn + 63) / 64) +
+                          (size_t)kLayers);
+}
 __global__ __launch_bounds__(kThreads) void k_layer_bins(const uint32_t* keys_in, const int32_t* perm_in,
                                                          const int32_t* bstart, const int32_t* bend, int nkeys,
                                                          int node_bits, int bin_x, uint32_t* keys_out,
                                                          int32_t* perm_out)
 {
     constexpr int kWaves = kThreads / 64;
-    __shared__ int wrun[kWaves][kMaxBinNodes];       // per wave and node: particles before the wave's share,
-                                                     // then the running count while the wave ranks its share
-    __shared__ int cnt[kMaxBinNodes];                // particles of each node
-    __shared__ int pre[kLayers][kMaxBinNodes];       // layer < last: nodes before n in the layer; last: particles
-    __shared__ int tot[kLayers][kMaxBinNodes / 64];  // per layer and node wave: nodes (or particles) it holds
-    __shared__ int loff[kLayers];                    // first position of each layer
+    // LDS sized by the launch for nn = 2^node_bits nodes (layer_bins_lds_bytes), so small bins leave room
+    // for more work groups per CU:
+    //   wrun[w * nn + n]   per wave w and node n: particles before the wave's share, then its running count
+    //   cnt[n]             particles of node n
+    //   pre[l * nn + n]    layer l < last: nodes before n in the layer; last: particles before n
+    //   tot[l * nw + v]    per layer and node wave v (nw = nn / 64): the nodes (or particles) it holds
+    //   loff[l]            first position of layer l
+    extern __shared__ int lds[];
+    const int nn = 1 << node_bits;
+    const int nw = (nn + 63) >> 6;
+    int* const wrun = lds;
+    int* const cnt = wrun + kWaves * nn;
+    int* const pre = cnt + nn;
+    int* const tot = pre + kLayers * nn;
+    int* const loff = tot + kLayers * nw;
     const int k = blockIdx.x;
     if (k >= nkeys) { return; }
     const int s = bstart[k], e = bend[k];
     if (e <= s) { return; }
-    const int nn = 1 << node_bits;
     const uint32_t mask = (uint32_t)nn - 1u;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -563,18 +578,18 @@ __global__ __launch_bounds__(kThreads) void k_layer_bins(const uint32_t* keys_in
     const int we = s + min((wave + 1) * gper * 64, len);
     for (int n = threadIdx.x; n < nn; n += kThreads) {
 #pragma unroll
-        for (int w = 0; w < kWaves; ++w) { wrun[w][n] = 0; }
+        for (int w = 0; w < kWaves; ++w) { wrun[w * nn + n] = 0; }
     }
     __syncthreads();
-    for (int j = ws + lane; j < we; j += 64) { atomicAdd(&wrun[wave][keys_in[j] & mask], 1); }
+    for (int j = ws + lane; j < we; j += 64) { atomicAdd(&wrun[wave * nn + (int)(keys_in[j] & mask)], 1); }
     __syncthreads();
     // per node: total, and the exclusive prefix over the waves' shares
     for (int n = threadIdx.x; n < nn; n += kThreads) {
         int c = 0;
 #pragma unroll
         for (int w = 0; w < kWaves; ++w) {
-            const int t = wrun[w][n];
-            wrun[w][n] = c;
+            const int t = wrun[w * nn + n];
+            wrun[w * nn + n] = c;
             c += t;
         }
         cnt[n] = c;
@@ -588,8 +603,8 @@ __global__ __launch_bounds__(kThreads) void k_layer_bins(const uint32_t* keys_in
 #pragma unroll
         for (int l = 0; l < kLayers - 1; ++l) {
             const uint64_t m = __ballot(c > l);
-            pre[l][n] = __popcll(m & lt_mask);
-            if (lane == 0) { tot[l][w] = __popcll(m); }
+            pre[l * nn + n] = __popcll(m & lt_mask);
+            if (lane == 0) { tot[l * nw + w] = __popcll(m); }
         }
         const int over = c > kLayers - 1 ? c - (kLayers - 1) : 0;
         int incl = over;
@@ -598,26 +613,25 @@ __global__ __launch_bounds__(kThreads) void k_layer_bins(const uint32_t* keys_in
             const int t = __shfl_up(incl, d, 64);
             if (lane >= d) { incl += t; }
         }
-        pre[kLayers - 1][n] = incl - over;
-        if (lane == 63) { tot[kLayers - 1][w] = incl; }
+        pre[(kLayers - 1) * nn + n] = incl - over;
+        if (lane == 63) { tot[(kLayers - 1) * nw + w] = incl; }
     }
     __syncthreads();
-    const int nw = (nn + 63) >> 6;
     for (int q = threadIdx.x; q < nn; q += kThreads) {
         const int n = checker_node(q, nn, bin_x);
         const int w = q >> 6;
 #pragma unroll
         for (int l = 0; l < kLayers; ++l) {
             int carry = 0;
-            for (int v = 0; v < w; ++v) { carry += tot[l][v]; }
-            pre[l][n] += carry;
+            for (int v = 0; v < w; ++v) { carry += tot[l * nw + v]; }
+            pre[l * nn + n] += carry;
         }
     }
     if (threadIdx.x == 0) {
         int o = 0;
         for (int l = 0; l < kLayers; ++l) {
             loff[l] = o;
-            for (int v = 0; v < nw; ++v) { o += tot[l][v]; }
+            for (int v = 0; v < nw; ++v) { o += tot[l * nw + v]; }
         }
     }
     __syncthreads();
@@ -637,12 +651,12 @@ __global__ __launch_bounds__(kThreads) void k_layer_bins(const uint32_t* keys_in
             peers &= bit ? m : ~m;
         }
         int r = 0;
-        if (valid) { r = wrun[wave][n] + __popcll(peers & lt_mask); }
+        if (valid) { r = wrun[wave * nn + n] + __popcll(peers & lt_mask); }
         __asm__ __volatile__("" ::: "memory");
-        if (valid && (peers >> lane) == 1ull) { wrun[wave][n] = r + 1; }
+        if (valid && (peers >> lane) == 1ull) { wrun[wave * nn + n] = r + 1; }
         __asm__ __volatile__("" ::: "memory");
-        const int off = r < kLayers - 1 ? loff[r] + pre[r][n]
-                                        : loff[kLayers - 1] + pre[kLayers - 1][n] + (r - (kLayers - 1));
+        const int off = r < kLayers - 1 ? loff[r] + pre[r * nn + n]
+                                        : loff[kLayers - 1] + pre[(kLayers - 1) * nn + n] + (r - (kLayers - 1));
         if (valid && off < len) {  // always true for a correct ranking; keeps a wrong one inside the bin
             perm_out[s + off] = perm_in[j];
             keys_out[s + off] = key >> node_bits;
@@ -705,13 +719,15 @@ hipError_t launch_binning(hipStream_t st, const DevParams& p, const Store& src, 
         vin = vout;
     }
     // kin / vin: keys sorted stably by bin and the permutation; bin ranges, then the layered order
+    if (layer_bins_lds_bytes(p.node_bits) > 65536) { return hipErrorInvalidValue; }
     hipError_t e = launch_fill_i32(st, ss.bin_start, (size_t)ss.num_keys, 0);
     if (e != hipSuccess) { return e; }
     e = launch_fill_i32(st, ss.bin_end, (size_t)ss.num_keys, 0);
     if (e != hipSuccess) { return e; }
     hipLaunchKernelGGL(k_bin_bounds, dim3(pblocks), dim3(kThreads), 0, st, kin, count, ss.num_keys, p.node_bits,
                        ss.bin_start, ss.bin_end);
-    hipLaunchKernelGGL(k_layer_bins, dim3((unsigned)ss.num_keys), dim3(kThreads), 0, st, kin, vin, ss.bin_start,
+    hipLaunchKernelGGL(k_layer_bins, dim3((unsigned)ss.num_keys), dim3(kThreads), layer_bins_lds_bytes(p.node_bits), st,
+                       kin, vin, ss.bin_start,
                        ss.bin_end, ss.num_keys, p.node_bits, p.bin_x, dst.key, ss.perm);
     hipLaunchKernelGGL(k_gather, dim3((pblocks + 7u) / 8u * 8u), dim3(kThreads), 0, st, src, dst, ss.perm, count);
     const unsigned kblocks = (unsigned)((ss.num_keys + kThreads - 1) / kThreads);
