@@ -434,6 +434,59 @@ def test_batched_scenes_isolated(oracle_mod):
         assert rel(out["grid"][0][k], grid[0]) < 1e-4, k
 
 
+@pytest.mark.parametrize("mode", MODES, ids=lambda m: MODE_NAMES[m])
+def test_empty_population_steps(mode):
+    """No particles: ticks run (a binning of nothing, no work items), the count stays 0, the grid and the
+    output slot stay empty, and a later emission simulates normally."""
+    p = small_params(substeps=4, mode=mode)
+    with Engine(p) as e:
+        e.set_particles(np.zeros((0, 2), np.float32))
+        e.step(2)
+        e.sync()
+        assert e.count() == 0
+        assert e.readback()["x"].shape[0] == 0
+        for g in e.read_grid():
+            assert not g.any()
+        assert e.read_view().count == 0
+        # a 10 x 10 block at the centre moving 0.02 per tick (50 per unit time: 0.3 cells per substep)
+        emitted = e.hose_pipe((0.5, 0.5), (0.02, 0.0), 0.02, 0.002, 0.0, 0)
+        assert emitted == 10 * 10
+        e.step(2)
+        e.sync()
+        out = e.readback()
+        assert out["x"].shape[0] == 100 and np.isfinite(out["x"]).all()
+        assert out["x"].min() >= 0.0 and out["x"].max() <= 1.0
+        assert e.read_grid()[0].sum() == pytest.approx(100 * p.mass, rel=1e-5)
+
+
+def test_ragged_batch_with_empty_and_single_particle_scenes(oracle_mod):
+    """Batched scenes of 400, 0, 1 and 2500 particles: each non-empty scene equals the FP64 oracle run
+    alone, and the empty scene's grid stays empty."""
+    p = abi.preset(abi.PRESET_BATCH64X256)
+    p.num_scenes = 4
+    p.substeps = 20
+    p.sort_interval = 10
+    counts = [400, 0, 1, 2500]
+    rng = np.random.default_rng(9)
+    x = np.concatenate([rng.uniform(0.35, 0.35 + 0.004 * np.sqrt(c), (c, 2)) for c in counts]).astype(np.float32)
+    n = x.shape[0]
+    v = rng.uniform(-1, 1, (n, 2)).astype(np.float32)
+    mat = rng.integers(0, 3, n).astype(np.uint8)
+    p.capacity = n
+    out = run_gpu(p, x, v, mat, scene_counts=np.array(counts, np.int32))
+    lo = 0
+    for k, c in enumerate(counts):
+        sl = slice(lo, lo + c)
+        lo += c
+        if c == 0:
+            assert not out["grid"][0][k].any()
+            continue
+        s, grid = run_oracle(oracle_mod, p, x[sl], v[sl], mat[sl], p.substeps, threads=4)
+        assert rel(out["x"][sl], s.x) < 1e-6, k
+        assert rel(out["v"][sl], s.v) < 1e-4, k
+        assert rel(out["grid"][0][k], grid[0]) < 1e-4, k
+
+
 def test_time_kernels_advances_like_step(oracle_mod):
     """The eager, event-timed launch sequence computes the same substeps as the captured tick graph."""
     p = small_params(substeps=10, mode=abi.MODE_FUSED, rho=4.0)
