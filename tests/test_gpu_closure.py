@@ -179,7 +179,8 @@ def test_learned_closure_requires_weights():
         assert ei.value.code == abi.MPM_ERR_STATE
 
 
-def test_learned_kernels_match_plain_on_fluid():
+@pytest.mark.parametrize("shape", [(8, 8), (16, 8), (16, 16), (32, 16)], ids=lambda s: f"{s[0]}x{s[1]}")
+def test_learned_kernels_match_plain_on_fluid(shape):
     """Fluid takes the analytic stress in the learned kernels too, so on an all-fluid scene the learned fused
     and P2G kernels must reproduce the plain ones up to the atomic-order noise of the grid flush.  The
     scene crowds whole waves onto single start nodes (1000 particles inside one cell), which drives the
@@ -200,7 +201,7 @@ def test_learned_kernels_match_plain_on_fluid():
     p.substeps = 10
     p.sort_interval = 5
     p.capacity = n
-    p.block_x, p.block_y = 16, 8
+    p.block_x, p.block_y = shape
     mat = np.full(n, abi.MAT_FLUID, np.uint8)
     outs = {}
     for name, cl in (("learned", abi.CLOSURE_LEARNED), ("plain", abi.CLOSURE_ANALYTIC)):

@@ -118,6 +118,34 @@ def test_crowded_start_nodes(oracle_mod, mode):
     assert rel(out["grid"][0][0], grid[0]) < 1e-5
 
 
+SHAPES = [(8, 8), (16, 8), (16, 16), (32, 16)]
+
+
+@pytest.mark.parametrize("mode", [abi.MODE_TILED, abi.MODE_FUSED], ids=lambda m: MODE_NAMES[m])
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: f"{s[0]}x{s[1]}")
+def test_every_tile_shape_against_oracle(oracle_mod, mode, shape):
+    """The tiled and fused kernels of every compiled tile shape (the autotuner may pick any of them) on a
+    crowded scene of all three materials: 700 particles inside one cell, 800 in a 2 x 2-cell cluster and
+    1500 spread over a block, random v and C, 10 substeps against the FP64 oracle."""
+    rng = np.random.default_rng(57)
+    cells = 64
+    one = (np.array([40.5, 28.5]) + rng.uniform(-0.3, 0.3, (700, 2))) / cells
+    four = (np.array([24.0, 36.0]) + rng.uniform(-0.9, 0.9, (800, 2))) / cells
+    spread = rng.uniform(0.25, 0.75, (1500, 2))
+    x = np.concatenate([one, four, spread]).astype(np.float32)
+    n = x.shape[0]
+    v = rng.uniform(-0.5, 0.5, (n, 2)).astype(np.float32)
+    C = rng.uniform(-1, 1, (n, 4)).astype(np.float32)
+    mat = rng.integers(0, 3, n).astype(np.uint8)
+    p = small_params(substeps=10, mode=mode, rho=4.0, block=shape)
+    out = run_gpu(p, x, v, mat, C=C)
+    s, grid = run_oracle(oracle_mod, p, x, v, mat, 10, threads=1, C=C)
+    assert rel(out["x"], s.x) < 1e-6
+    assert rel(out["v"], s.v) < 1e-4
+    assert rel(out["F"], s.F) < 1e-5
+    assert rel(out["grid"][0][0], grid[0]) < 1e-5
+
+
 @pytest.mark.parametrize("mode", MODES, ids=lambda m: MODE_NAMES[m])
 @pytest.mark.parametrize("substeps", [1, 4])
 def test_forced_failures_against_oracle(oracle_mod, mode, substeps):
