@@ -262,3 +262,51 @@ def render(pos, width, height, chunk_px=160, scale=5.0, flip_extent=0.0):
     img = np.zeros((height, width, 4), np.uint8)
     lib().oracle_render(pos.shape[0], _p(pos, _f32), width, height, chunk_px, scale, flip_extent, _p(img, _u8))
     return img
+
+
+def philox4x32_10(counters, key):
+    """Philox4x32-10 (Salmon, Moraes, Dror, Shaw 2011, "Parallel random numbers: as easy as 1, 2, 3"):
+    ten rounds of two 32x32->64-bit products with the multipliers 0xD2511F53 and 0xCD9E8D57, the key bumped by
+    the Weyl constants 0x9E3779B9 and 0xBB67AE85 after each round.  counters: uint32 [n, 4]; key: two uint32.
+    Returns uint32 [n, 4].  The reference's emitter draws its jitter from cuRAND's Philox4_32_10
+    (MPM.cu:378, 416-418); the engine's emitter uses this generator with its own counter layout."""
+    c = np.ascontiguousarray(counters, dtype=np.uint64).reshape(-1, 4)
+    k0, k1 = np.uint64(int(key[0]) & 0xFFFFFFFF), np.uint64(int(key[1]) & 0xFFFFFFFF)
+    m0, m1, lo = np.uint64(0xD2511F53), np.uint64(0xCD9E8D57), np.uint64(0xFFFFFFFF)
+    for _ in range(10):
+        p0 = m0 * c[:, 0]
+        p1 = m1 * c[:, 2]
+        c = np.stack([(p1 >> np.uint64(32)) ^ c[:, 1] ^ k0, p1 & lo, (p0 >> np.uint64(32)) ^ c[:, 3] ^ k1, p0 & lo],
+                     axis=1)
+        k0 = (k0 + np.uint64(0x9E3779B9)) & lo
+        k1 = (k1 + np.uint64(0xBB67AE85)) & lo
+    return c.astype(np.uint32)
+
+
+def hose_pipe_positions(origin, velocity, width, rows, spacing, jitter, tick):
+    """Positions of one hose-pipe emission in float64 (MPMEmitterKernel, MPM.cu:380-438, with the engine's
+    generator): particle i = row * width + col sits on a width x rows lattice trailing the nozzle,
+    across = (col - (width - 1) / 2) spacing along the nozzle normal n = (-t_y, t_x) and
+    along = -(row + 1/2) spacing along the stream direction t, plus a Gaussian jitter: Philox4x32-10 on the
+    counter (i, 0, 0, 0) with the key (tick low word, tick high word), u1 = ((r0 >> 8) + 1) / 2^24,
+    u2 = (r1 >> 8) / 2^24 and Box-Muller (sqrt(-2 ln u1) cos 2 pi u2, sqrt(-2 ln u1) sin 2 pi u2) * jitter.
+    Returns (lattice, jitter offsets), both float64 [width * rows, 2]."""
+    n = width * rows
+    vx, vy = float(np.float32(velocity[0])), float(np.float32(velocity[1]))
+    sp = np.hypot(vx, vy)
+    tx, ty = (vx / sp, vy / sp) if sp > 0 else (0.0, 0.0)
+    i = np.arange(n)
+    across = (i % width - 0.5 * (width - 1)) * float(np.float32(spacing))
+    along = -((i // width) + 0.5) * float(np.float32(spacing))
+    lattice = np.stack([origin[0] - ty * across + tx * along, origin[1] + tx * across + ty * along], axis=1)
+    off = np.zeros((n, 2))
+    if jitter > 0:
+        ctr = np.zeros((n, 4), np.uint32)
+        ctr[:, 0] = i
+        r = philox4x32_10(ctr, (tick & 0xFFFFFFFF, tick >> 32))
+        u1 = ((r[:, 0] >> 8).astype(np.float64) + 1.0) / 16777216.0
+        u2 = (r[:, 1] >> 8).astype(np.float64) / 16777216.0
+        rad = np.sqrt(-2.0 * np.log(u1))
+        off[:, 0] = rad * np.cos(2.0 * np.pi * u2) * float(np.float32(jitter))
+        off[:, 1] = rad * np.sin(2.0 * np.pi * u2) * float(np.float32(jitter))
+    return lattice, off

@@ -455,6 +455,32 @@ def test_output_view_and_emitter():
         assert view.slot == 1 and view.count == 720
 
 
+def test_emitter_jitter_matches_philox_box_muller(oracle_mod):
+    """A 100 x 100 hose-pipe emission with jitter 0.25 (reference configuration): every particle sits at its
+    lattice point plus the Box-Muller jitter of the oracle's Philox4x32-10 (pinned by the Random123 known
+    answers; its normality is tested on a million draws on the CPU) within FP32 rounding; the offsets divided
+    by the jitter have mean 0 and standard deviation 1 within 5 standard errors, uncorrelated axes, and
+    another tick draws an independent jitter."""
+    p = abi.preset(abi.PRESET_REFERENCE)
+    origin, vel, nozzle, spacing, jitter = (200.0, 96.0), (100.0, 0.0), 100.0, 1.0, 0.25
+    offs = []
+    with Engine(p) as e:
+        e.set_particles(np.zeros((0, 2), np.float32))
+        for tick in (7, 8):
+            assert e.hose_pipe(origin, vel, nozzle, spacing, jitter, tick) == 100 * 100
+            x = e.readback()["x"][-10000:].astype(np.float64)
+            lattice, off = oracle_mod.hose_pipe_positions(origin, vel, 100, 100, spacing, jitter, tick)
+            want = lattice + off
+            ulp = np.spacing(np.abs(want).astype(np.float32)).astype(np.float64)
+            assert np.all(np.abs(x - want) <= 3 * ulp + 2e-5 * jitter), tick
+            z = (x - lattice) / jitter
+            assert np.all(np.abs(z.mean(axis=0)) < 5 / np.sqrt(10000)), tick
+            assert np.all(np.abs(z.std(axis=0) - 1) < 5 / np.sqrt(2 * 10000)), tick
+            assert abs(np.corrcoef(z[:, 0], z[:, 1])[0, 1]) < 5 / np.sqrt(10000)
+            offs.append(z)
+    assert abs(np.corrcoef(offs[0][:, 0], offs[1][:, 0])[0, 1]) < 5 / np.sqrt(10000)
+
+
 def test_capacity_is_respected():
     p = abi.preset(abi.PRESET_REFERENCE)
     p.capacity = 300

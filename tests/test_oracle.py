@@ -484,3 +484,32 @@ def test_adam_first_step_closed_form(oracle_mod):
     mh = m2 / (1 - 0.9 ** 2)
     vh = v2 / (1 - 0.999 ** 2)
     assert np.allclose(w, w1 - 1e-3 * mh / (np.sqrt(vh) + 1e-8), rtol=1e-12, atol=1e-15)
+
+
+def test_philox_known_answer_vectors(oracle_mod):
+    """The Philox4x32-10 restatement against the published known-answer vectors of the Random123 library
+    (Salmon et al. 2011): zero counter and key, all-ones counter and key, and the pi-digits counter/key."""
+    kat = [((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+           ((0xFFFFFFFF,) * 4, (0xFFFFFFFF, 0xFFFFFFFF), (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+           ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+            (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1))]
+    for ctr, key, want in kat:
+        got = oracle_mod.philox4x32_10(np.array([ctr], np.uint32), key)[0]
+        assert [int(g) for g in got] == list(want)
+
+
+def test_hose_pipe_jitter_is_standard_normal(oracle_mod):
+    """Box-Muller on the Philox outputs of a million emitted particles: each axis passes a Kolmogorov-Smirnov
+    test against N(0, jitter^2), mean and standard deviation within 5 standard errors, axes uncorrelated."""
+    from scipy import stats
+    lattice, off = oracle_mod.hose_pipe_positions((0.0, 0.0), (1000.0, 0.0), 1000, 1000, 1.0, 0.5, 7)
+    z = off / 0.5
+    n = z.shape[0]
+    for axis in (0, 1):
+        assert stats.kstest(z[:, axis], "norm").pvalue > 1e-3
+    assert np.all(np.abs(z.mean(axis=0)) < 5 / np.sqrt(n))
+    assert np.all(np.abs(z.std(axis=0) - 1) < 5 / np.sqrt(2 * n))
+    assert abs(np.corrcoef(z[:, 0], z[:, 1])[0, 1]) < 5 / np.sqrt(n)
+    # the lattice: 1000 columns across the nozzle normal (+y), rows trailing along -x at half-spacing offsets
+    assert np.allclose(lattice[:1000, 1], np.arange(1000) - 499.5) and np.allclose(lattice[::1000, 0], -(np.arange(1000) + 0.5))
+
