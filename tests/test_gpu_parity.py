@@ -293,7 +293,8 @@ def test_reference_preset_against_oracle(oracle_mod, mode):
 def test_binning_bit_exact(oracle_mod, shape):
     """The engine's binning equals the layered order of the oracle (stable sort by FP32 bin key and stencil
     node, then layers of one particle per node, nodes in checkerboard order) bit for bit, including a
-    shuffled input and several scenes."""
+    shuffled input and several scenes.  Scene 2 holds 3000 particles inside one cell (ranks in the thousands
+    at one node) and 5000 over a 10 x 10-cell patch (bins in which every wave ranks several groups)."""
     p = abi.preset(abi.PRESET_BATCH64X256)
     p.num_scenes = 4
     p.block_x, p.block_y = shape
@@ -304,6 +305,8 @@ def test_binning_bit_exact(oracle_mod, shape):
     x = rng.uniform(0, 1, (n, 2)).astype(np.float32)
     x[:50] = 1.0
     x[50:100] = 0.0
+    x[5001:8001] = ((np.array([100.5, 60.5]) + rng.uniform(-0.3, 0.3, (3000, 2))) / 256).astype(np.float32)
+    x[8001:13001] = ((np.array([40.0, 150.0]) + rng.uniform(0, 10, (5000, 2))) / 256).astype(np.float32)
     scene_of = np.repeat(np.arange(4), counts).astype(np.int32)
     with Engine(p) as e:
         e.set_particles(x, material=np.ones(n, np.uint8), scene_counts=counts)
