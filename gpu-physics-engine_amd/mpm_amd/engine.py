@@ -12,6 +12,8 @@ Engine.load_state).  Every call raises MpmError on a
 negative status; there is no fallback path.
 """
 import ctypes
+import json
+import os
 
 import numpy as np
 
@@ -198,11 +200,65 @@ class Engine:
         return tau
 
 
-def autotune(params, scene, substeps=None):
+def tune_key(params, scene, substeps=None):
+    """What an autotuned launch shape depends on: grid, scenes, particle count (to the nearest power of two),
+    the set of materials, mode, closure and substeps per timed tick.  A string, the key of the tune cache."""
+    n = int(np.asarray(scene.x).reshape(-1, 2).shape[0])
+    mats = sorted(int(m) for m in np.unique(np.asarray(scene.material)))
+    return json.dumps({"grid": [int(params.grid_w), int(params.grid_h)], "scenes": int(params.num_scenes),
+                       "particles_log2": int(round(np.log2(max(n, 1)))), "materials": mats,
+                       "mode": int(params.mode), "closure": int(params.closure),
+                       "substeps": int(substeps or params.substeps)}, sort_keys=True)
+
+
+class TuneCache:
+    """Autotuner choices kept in a small JSON file: {tune_key: {block_x, block_y, chunk, sort_interval,
+    ms_per_substep}}.  Writes go to a temporary file that replaces the cache in one rename, so a reader
+    never sees a half-written file; an unreadable file counts as empty."""
+
+    FIELDS = ("block_x", "block_y", "chunk", "sort_interval", "ms_per_substep")
+
+    def __init__(self, path):
+        self.path = str(path)
+
+    def _load(self):
+        try:
+            with open(self.path) as fh:
+                data = json.load(fh)
+            return data if isinstance(data, dict) else {}
+        except (OSError, ValueError):
+            return {}
+
+    def get(self, key):
+        entry = self._load().get(key)
+        if not isinstance(entry, dict) or any(k not in entry for k in self.FIELDS):
+            return None
+        return entry
+
+    def put(self, key, res):
+        data = self._load()
+        data[key] = {k: (float(getattr(res, k)) if k == "ms_per_substep" else int(getattr(res, k)))
+                     for k in self.FIELDS}
+        tmp = f"{self.path}.tmp{os.getpid()}"
+        with open(tmp, "w") as fh:
+            json.dump(data, fh, indent=1, sort_keys=True)
+        os.replace(tmp, self.path)
+
+
+def autotune(params, scene, substeps=None, cache=None):
     """Run the engine's autotuner (mpm_autotune) on a scene; returns a tuned copy of params and the
-    MpmTuneResult."""
-    lib = abi.load_library()
+    MpmTuneResult.  With cache (a file path), a choice stored for the same tune_key is returned without
+    tuning (the result's candidates field is then 0), and a new choice is stored."""
     p = params.copy()
+    key = tune_key(params, scene, substeps) if cache else None
+    if cache:
+        hit = TuneCache(cache).get(key)
+        if hit is not None:
+            res = abi.MpmTuneResult(int(hit["block_x"]), int(hit["block_y"]), int(hit["chunk"]),
+                                    int(hit["sort_interval"]), float(hit["ms_per_substep"]), 0)
+            p.block_x, p.block_y, p.chunk, p.sort_interval = res.block_x, res.block_y, res.chunk, res.sort_interval
+            return p, res
+    lib = abi.load_library()
     x = np.ascontiguousarray(scene.x, dtype=np.float32)
     v = np.ascontiguousarray(scene.v, dtype=np.float32)
     m = np.ascontiguousarray(scene.material, dtype=np.uint8)
@@ -215,4 +271,6 @@ def autotune(params, scene, substeps=None):
         msg = lib.mpm_last_message(None)
         raise MpmError(rc, f"mpm_autotune: {msg.decode() if msg else ''}")
     p.block_x, p.block_y, p.chunk, p.sort_interval = res.block_x, res.block_y, res.chunk, res.sort_interval
+    if cache:
+        TuneCache(cache).put(key, res)
     return p, res

@@ -607,18 +607,24 @@ def test_graph_replays_match_eager():
             g.readback()  # eager host work between replays
 
 
-def test_autotune_picks_a_valid_configuration(oracle_mod):
+def test_autotune_picks_a_valid_configuration(oracle_mod, tmp_path):
     """mpm_autotune times every compiled tile shape x chunk on the caller's scene and returns one of them;
-    the tuned engine still matches the FP64 oracle."""
+    the choice is stored in the tune cache and returned from it on the next call; the tuned engine still
+    matches the FP64 oracle."""
     from mpm_amd.engine import autotune
     p = abi.preset(abi.PRESET_MIXED1024)
     p.substeps = 20
     p.sort_interval = 10
     sc = scenes.mixed(n=65536)
-    tuned, res = autotune(p, sc)
+    cache = tmp_path / "tune.json"
+    tuned, res = autotune(p, sc, cache=cache)
     assert (res.block_x, res.block_y) in [(8, 8), (16, 8), (16, 16), (32, 16)]
     assert res.chunk in (512, 1024, 2048, 4096) and res.candidates >= 16
     assert res.ms_per_substep > 0
+    again, res2 = autotune(p, sc, cache=cache)
+    assert res2.candidates == 0
+    assert (again.block_x, again.block_y, again.chunk, again.sort_interval) == \
+        (tuned.block_x, tuned.block_y, tuned.chunk, tuned.sort_interval)
     out = run_gpu(tuned, sc.x, sc.v, sc.material)
     s, _ = run_oracle(oracle_mod, tuned, sc.x, sc.v, sc.material, tuned.substeps)
     assert rel(out["x"], s.x) < 1e-6
