@@ -144,6 +144,8 @@ def main():
     ap.add_argument("--autotune", action="store_true", help="pick tile shape, chunk and sort interval by mpm_autotune")
     ap.add_argument("--sort-interval", type=int, default=0, metavar="SUBSTEPS",
                     help="substeps between binnings (0 = the preset's; above the tick's substeps: every k-th tick)")
+    ap.add_argument("--chunk", type=int, default=0, metavar="PARTICLES",
+                    help="particles per work item (0 = the preset's)")
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="mixed1024",
                     help="mixed1024 is the headline; the others are the remaining configurations")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -161,6 +163,8 @@ def main():
     p.mode = {"reference": abi.MODE_REFERENCE, "tiled": abi.MODE_TILED, "fused": abi.MODE_FUSED}[args.mode]
     if args.sort_interval > 0:
         p.sort_interval = args.sort_interval
+    if args.chunk > 0:
+        p.chunk = args.chunk
     scene = make_scene(args.scale)
     p.capacity = scene.n
     log(f"scene: {scene.description}")
