@@ -54,8 +54,10 @@ __device__ __forceinline__ float fast_atan2(float y, float x)
 // Closure features of one particle; returns false when outside the trained range (then x is unspecified).
 __device__ __forceinline__ bool closure_features(const Mat2& F, float jh, float x[4])
 {
-    float uc, us, s0, s1, vc, vs;
-    svd2(F, uc, us, s0, s1, vc, vs);
+    float uc, us, s0, s1, vc, vs, rc, rs;
+    svd2(F, uc, us, s0, s1, vc, vs, rc, rs);
+    (void)rc;  // the features use U and the singular values only
+    (void)rs;
     // canonical order s0 >= s1: swapping the singular values rotates U by 90 degrees (tau is unchanged)
     float th = fast_atan2(us, uc);
     if (s1 > s0) {

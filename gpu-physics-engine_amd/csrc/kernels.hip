@@ -251,9 +251,11 @@ __global__ __launch_bounds__(kThreads) void k_g2p_reference(DevParams p, Store s
             B.a11 += wvy * dpy;
         }
     }
-    Mat2 C, F = load_F(s, i);
+    const float sB = 4.f * p.inv_dx * p.inv_dx;  // C = 4 inv_dx^2 B, MPM.cu:255
+    const Mat2 C = {sB * B.a00, sB * B.a01, sB * B.a10, sB * B.a11};
+    Mat2 F = load_F(s, i);
     float J = s.J[i];
-    g2p_update(p, s.mat[i], px, py, vx, vy, B, C, F, J);
+    g2p_update(p, s.mat[i], px, py, vx, vy, C, F, J);
     s.x[i] = px;
     s.y[i] = py;
     s.vx[i] = vx;
@@ -905,7 +907,8 @@ __device__ __forceinline__ void flush_acc_tiles(const DevParams& p, const Grid& 
 // Gather of one particle (MPM.cu:227-251) from the LDS velocity tile, or from HBM when it drifted out of
 // the tile.  v = sum w_ab v_ab, and with d_ab = ((a - fx) dx, (b - fy) dx) the affine moment
 // B = sum w_ab v_ab (x) d_ab = dx (sum w_ab v_ab (a, b) - v (x) f): two first moments replace the
-// per-node outer products (each column of three nodes is reduced with wy first).
+// per-node outer products (each column of three nodes is reduced with wy first).  The gather returns
+// C = 4 inv_dx^2 B (MPM.cu:255) = 4 inv_dx (sum w_ab v_ab (a, b) - v (x) f), one scale per entry.
 template <class Load>
 __device__ __forceinline__ void gather_moments(const float wx[3], const float wy[3], Load load, float& sx, float& sy,
                                                float& ax, float& ay, float& bx, float& by)
@@ -974,7 +977,7 @@ __device__ __forceinline__ void gather_moments_hbm(const DevParams& p, const Gri
 
 template <int BX, int BY>
 __device__ __forceinline__ void gather(const DevParams& p, const Grid& g, size_t gb, int ox, int oy, const float2* sv,
-                                       float px, float py, float& vx, float& vy, Mat2& B)
+                                       float px, float py, float& vx, float& vy, Mat2& C)
 {
     using T = TileShape<BX, BY>;
     int bx, by;
@@ -994,7 +997,9 @@ __device__ __forceinline__ void gather(const DevParams& p, const Grid& g, size_t
     }
     vx = sx;
     vy = sy;
-    B = {p.dx * (mx - fx * sx), p.dx * (nx - fy * sx), p.dx * (my - fx * sy), p.dx * (ny - fy * sy)};
+    // C = 4 inv_dx^2 B with B = dx (first moments - v (x) f): one scale 4 inv_dx per entry
+    const float sC = 4.f * p.inv_dx;
+    C = {sC * (mx - fx * sx), sC * (nx - fy * sx), sC * (my - fx * sy), sC * (ny - fy * sy)};
 }
 
 // Scatter of one particle into its wave's LDS accumulator tile, without atomics, in rounds in which the
@@ -1150,10 +1155,10 @@ __global__ __launch_bounds__(kThreads) void k_g2p_tiled(DevParams p, Store s, Gr
         const uint32_t o = (uint32_t)j * 4u;
         float px = ldo(s.x, o), py = ldo(s.y, o), vx, vy;
         const int mat = ldo(s.mat, (uint32_t)j);
-        Mat2 C, B, F = ld_F(s, o);
+        Mat2 C, F = ld_F(s, o);
         float J = ldo(s.J, o);
-        gather<BX, BY>(p, g, gb, ox, oy, sv, px, py, vx, vy, B);
-        g2p_update(p, mat, px, py, vx, vy, B, C, F, J);
+        gather<BX, BY>(p, g, gb, ox, oy, sv, px, py, vx, vy, C);
+        g2p_update(p, mat, px, py, vx, vy, C, F, J);
         sto(s.x, o, px);
         sto(s.y, o, py);
         sto(s.vx, o, vx);
@@ -1207,15 +1212,16 @@ k_g2p2g(
             ny = ldo(s.y, o + 4u * kThreads);
         }
         const int mat = ldo(s.mat, (uint32_t)j);
-        Mat2 C, B, F = ld_F(s, o);
+        Mat2 C, F = ld_F(s, o);
         float J = ldo(s.J, o);
-        gather<BX, BY>(p, g, gb, ox, oy, sv, px, py, vx, vy, B);
-        g2p_update(p, mat, px, py, vx, vy, B, C, F, J);
+        gather<BX, BY>(p, g, gb, ox, oy, sv, px, py, vx, vy, C);
+        float2 snow_rot = make_float2(1.f, 0.f);  // a snow particle's polar rotation, from its return mapping
+        g2p_update(p, mat, px, py, vx, vy, C, F, J, &snow_rot);
         sto(s.x, o, px);
         sto(s.y, o, py);
         st_F(s, o, F);
         sto(s.J, o, J);
-        const Mat2 Q = affine_q(p, mat, C, F, J);
+        const Mat2 Q = affine_q(p, mat, C, F, J, &snow_rot);
         scatter<BX, BY>(p, g, gb, ox, oy, wtile, px, py, vx, vy, Q);
     }
     __syncthreads();
@@ -1309,10 +1315,10 @@ k_g2p2g_learned(DevParams p, Store s, Grid g, const Item* items,
         const uint32_t o = (uint32_t)(valid ? j : item.end - 1) * 4u;
         float px = ldo(s.x, o), py = ldo(s.y, o), vx, vy;
         const int mat = ldo(s.mat, o >> 2);
-        Mat2 C, B, F = ld_F(s, o);
+        Mat2 C, F = ld_F(s, o);
         float J = ldo(s.J, o);
-        gather<BX, BY>(p, g, gb, ox, oy, sv, px, py, vx, vy, B);
-        g2p_update(p, mat, px, py, vx, vy, B, C, F, J);
+        gather<BX, BY>(p, g, gb, ox, oy, sv, px, py, vx, vy, C);
+        g2p_update(p, mat, px, py, vx, vy, C, F, J);
         if (valid) {
             sto(s.x, o, px);
             sto(s.y, o, py);

@@ -168,13 +168,11 @@ __device__ __forceinline__ void polar_cs(const Mat2& F, float& c, float& s)
 }
 
 // Fixed-corotated Kirchhoff stress tau = 2 mu (F - R) F^T + lambda (J - 1) J I, J = det F
-// (Stomakhin et al. 2012).  (F - R) F^T = F F^T - R F^T and R F^T = R S R^T is symmetric, so tau is
-// symmetric; returns tau00, tau01 (= tau10), tau11.
-__device__ __forceinline__ void corotated_tau(const Mat2& F, float mu, float lambda, float& t00, float& t01,
-                                              float& t11)
+// (Stomakhin et al. 2012), with R = [[c, -s], [s, c]] the polar rotation of F.  (F - R) F^T = F F^T - R F^T
+// and R F^T = R S R^T is symmetric, so tau is symmetric; returns tau00, tau01 (= tau10), tau11.
+__device__ __forceinline__ void corotated_tau_r(const Mat2& F, float c, float s, float mu, float lambda, float& t00,
+                                                float& t01, float& t11)
 {
-    float c, s;
-    polar_cs(F, c, s);
     const float J = det2(F);
     // D = F - R, R = [[c, -s], [s, c]]
     const float d00 = F.a00 - c, d01 = F.a01 + s, d10 = F.a10 - s, d11 = F.a11 - c;
@@ -187,14 +185,23 @@ __device__ __forceinline__ void corotated_tau(const Mat2& F, float mu, float lam
     t01 = two_mu * m01;
     t11 = two_mu * m11 + vol;
 }
-
-// Signed SVD F = U diag(s0, s1) V^T with U, V rotations: polar F = R S, then one Jacobi rotation
-// diagonalising the symmetric S.  Outputs cos/sin of U and V.
-__device__ __forceinline__ void svd2(const Mat2& F, float& uc, float& us, float& s0, float& s1, float& vc,
-                                     float& vs)
+__device__ __forceinline__ void corotated_tau(const Mat2& F, float mu, float lambda, float& t00, float& t01,
+                                              float& t11)
 {
     float c, s;
     polar_cs(F, c, s);
+    corotated_tau_r(F, c, s, mu, lambda, t00, t01, t11);
+}
+
+// Signed SVD F = U diag(s0, s1) V^T with U, V rotations: polar F = R S, then one Jacobi rotation
+// diagonalising the symmetric S.  Outputs cos/sin of U and V, and of R (= U V^T).
+__device__ __forceinline__ void svd2(const Mat2& F, float& uc, float& us, float& s0, float& s1, float& vc,
+                                     float& vs, float& rc, float& rs)
+{
+    float c, s;
+    polar_cs(F, c, s);
+    rc = c;
+    rs = s;
     // S = R^T F, R = [[c, -s], [s, c]]
     const float S00 = c * F.a00 + s * F.a10;
     const float S01 = c * F.a01 + s * F.a11;
@@ -221,11 +228,13 @@ __device__ __forceinline__ void svd2(const Mat2& F, float& uc, float& us, float&
 }
 
 // Snow return mapping (Stomakhin et al. 2013): clamp the singular values of the trial elastic F to
-// [1 - theta_c, 1 + theta_s], move the removed volume into Jp, rebuild F = U diag(s') V^T.
-__device__ __forceinline__ void snow_project(const DevParams& p, Mat2& F, float& Jp)
+// [1 - theta_c, 1 + theta_s], move the removed volume into Jp, rebuild F = U diag(s') V^T.  (rc, rs) is the
+// polar rotation of the trial F, R = U V^T; the clamped singular values are positive, so it is also the
+// polar rotation of the rebuilt F.
+__device__ __forceinline__ void snow_project(const DevParams& p, Mat2& F, float& Jp, float& rc, float& rs)
 {
     float uc, us, s0, s1, vc, vs;
-    svd2(F, uc, us, s0, s1, vc, vs);
+    svd2(F, uc, us, s0, s1, vc, vs, rc, rs);
     const float lo = 1.f - p.theta_c, hi = 1.f + p.theta_s;
     const float n0 = fminf(fmaxf(s0, lo), hi);
     const float n1 = fminf(fmaxf(s1, lo), hi);
@@ -239,14 +248,18 @@ __device__ __forceinline__ void snow_project(const DevParams& p, Mat2& F, float&
     F.a11 = u10 * n0 * v10 + u11 * n1 * v11;
 }
 
-// Affine momentum matrix Q of P2G.  Fluid follows the reference expression order exactly
+// Affine momentum matrix Q of P2G.  Fluid follows the reference expression order
 // (MPM.cu:68-80: Tait pressure on J, V = V0 J, coeff = -dt V 4 inv_dx^2, Q = m C + coeff sigma);
 // jelly and snow use Q = m C - dt V0 4 inv_dx^2 tau with tau the fixed-corotated Kirchhoff stress.
-__device__ __forceinline__ Mat2 affine_q(const DevParams& p, int mat, const Mat2& C, const Mat2& F, float J)
+// snow_rot: for a snow particle, the polar rotation of F when the caller holds it from the return mapping
+// (g2p_update), else nullptr and it is computed here.
+__device__ __forceinline__ Mat2 affine_q(const DevParams& p, int mat, const Mat2& C, const Mat2& F, float J,
+                                         const float2* snow_rot = nullptr)
 {
     if (mat == MPM_MAT_FLUID) {
-        // (1/J)^gamma; gamma = 1 (linear bulk modulus) skips powf, whose result would be 1/J exactly.
-        const float inv_j = 1.f / J;
+        // (1/J)^gamma; gamma = 1 (linear bulk modulus) skips powf, whose result would be 1/J exactly.  1/J is
+        // the hardware reciprocal (1 ulp; every G2P clamps J to [j_min, j_max], j_min > 0).
+        const float inv_j = __builtin_amdgcn_rcpf(J);
         const float p_hat = p.tait_k * ((p.tait_gamma == 1.f ? inv_j : powf(inv_j, p.tait_gamma)) - 1.f);
         const float pressure = fminf(fmaxf(p_hat, p.p_min), p.p_max);
         const float V = p.vol * J;
@@ -261,8 +274,15 @@ __device__ __forceinline__ Mat2 affine_q(const DevParams& p, int mat, const Mat2
         mu *= h;
         lambda *= h;
     }
+    float c, s;
+    if (mat == MPM_MAT_SNOW && snow_rot) {
+        c = snow_rot->x;
+        s = snow_rot->y;
+    } else {
+        polar_cs(F, c, s);
+    }
     float t00, t01, t11;
-    corotated_tau(F, mu, lambda, t00, t01, t11);
+    corotated_tau_r(F, c, s, mu, lambda, t00, t01, t11);
     const float D = p.dt * p.vol * 4.f * p.inv_dx * p.inv_dx;
     return {p.mass * C.a00 - D * t00, p.mass * C.a01 - D * t01, p.mass * C.a10 - D * t01, p.mass * C.a11 - D * t11};
 }
@@ -289,20 +309,22 @@ __device__ __forceinline__ void handle_oob(const DevParams& p, float& px, float&
     oob_axis(py, vy, p.sim_h);
 }
 
-// Particle update after the gather (MPM.cu:252-265): advect, C = 4 inv_dx^2 B (before the OOB handling),
-// out-of-bounds, F <- (I + dt C) F, then J <- clamp(J det A) or, for snow, the return mapping on F and Jp.
+// Particle update after the gather (MPM.cu:252-265): advect, out-of-bounds, F <- (I + dt C) F, then
+// J <- clamp(J det A) or, for snow, the return mapping on F and Jp.  C = 4 inv_dx^2 B (MPM.cu:255) comes from
+// the caller's gather (formed before the OOB handling, as in the reference).  For snow, *snow_rot (if given)
+// receives the polar rotation of the new F, for affine_q.
 __device__ __forceinline__ void g2p_update(const DevParams& p, int mat, float& px, float& py, float& vx, float& vy,
-                                           const Mat2& B, Mat2& C, Mat2& F, float& J)
+                                           const Mat2& C, Mat2& F, float& J, float2* snow_rot = nullptr)
 {
-    const float sB = 4.f * p.inv_dx * p.inv_dx;
     px += vx * p.dt;
     py += vy * p.dt;
-    C = {sB * B.a00, sB * B.a01, sB * B.a10, sB * B.a11};
     handle_oob(p, px, py, vx, vy);
     const Mat2 A = {1.f + C.a00 * p.dt, C.a01 * p.dt, C.a10 * p.dt, 1.f + C.a11 * p.dt};
     F = mul2(A, F);
     if (mat == MPM_MAT_SNOW) {
-        snow_project(p, F, J);
+        float rc, rs;
+        snow_project(p, F, J, rc, rs);
+        if (snow_rot) { *snow_rot = make_float2(rc, rs); }
     } else {
         J *= det2(A);
         J = fminf(fmaxf(J, p.j_min), p.j_max);
