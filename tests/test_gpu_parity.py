@@ -359,6 +359,20 @@ def test_modes_agree_full_size():
         assert m == pytest.approx(sc.n * p.mass, rel=1e-4)
 
 
+def test_mixed1024_long_run_stays_finite_and_conserves_mass():
+    """The 4,194,304-particle mixed scene for 60 ticks (6000 substeps, binning every second tick, the boxes
+    falling and piling on the floor): every particle stays finite and inside the domain, none is lost, and
+    the grid mass of the last substep equals N m."""
+    p = abi.preset(abi.PRESET_MIXED1024)
+    sc = scenes.mixed()
+    out = run_gpu(p, sc.x, sc.v, sc.material, ticks=60)
+    assert out["x"].shape[0] == sc.n
+    assert np.all(np.isfinite(out["x"])) and np.all(np.isfinite(out["v"])) and np.all(np.isfinite(out["F"]))
+    assert out["x"].min() >= 0.0 and out["x"].max() <= 1.0
+    assert np.all(out["J"] > 0.0)
+    assert out["grid"][0].astype(np.float64).sum() == pytest.approx(sc.n * p.mass, rel=1e-4)
+
+
 def test_batch64x256_full_size(oracle_mod):
     """The full 64-scene batch (7,046,125 particles, counts log-uniform in [4096, 524288]) for one tick of
     200 substeps: every scene conserves its mass, every particle stays finite and in the domain, and the
