@@ -122,19 +122,24 @@ class Engine:
         _check(self.lib, self.h, self.lib.mpm_count(self.h, ctypes.byref(n)), "mpm_count")
         return n.value
 
-    def readback(self):
+    def readback(self, fields=("x", "v", "C", "F", "J", "material")):
+        """Host copy of the particle state in caller order; `fields` names the arrays to copy (the others are
+        passed to mpm_readback as NULL and left out of the result)."""
         n = self.count()
-        x = np.zeros((n, 2), np.float32)
-        v = np.zeros((n, 2), np.float32)
-        C = np.zeros((n, 4), np.float32)
-        F = np.zeros((n, 4), np.float32)
-        J = np.zeros(n, np.float32)
-        m = np.zeros(n, np.uint8)
+        shapes = {"x": ((n, 2), np.float32), "v": ((n, 2), np.float32), "C": ((n, 4), np.float32),
+                  "F": ((n, 4), np.float32), "J": ((n,), np.float32), "material": ((n,), np.uint8)}
+        unknown = set(fields) - set(shapes)
+        if unknown:
+            raise ValueError(f"unknown readback fields {sorted(unknown)}")
+        out = {k: np.zeros(*shapes[k]) for k in fields}
+        m = out.get("material")
         got = ctypes.c_int32()
-        rc = self.lib.mpm_readback(self.h, _fp(x), _fp(v), _fp(C), _fp(F), _fp(J),
-                                   m.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), ctypes.byref(got))
+        rc = self.lib.mpm_readback(self.h, _fp(out.get("x")), _fp(out.get("v")), _fp(out.get("C")),
+                                   _fp(out.get("F")), _fp(out.get("J")),
+                                   None if m is None else m.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                   ctypes.byref(got))
         _check(self.lib, self.h, rc, "mpm_readback")
-        return {"x": x, "v": v, "C": C, "F": F, "J": J, "material": m}
+        return out
 
     def read_grid(self):
         p = self.params

@@ -1,0 +1,136 @@
+"""Acceptance runs at the full size of the named workloads, through the C-ABI, against the FP64 oracle.
+
+- mixed1024: 4,194,304 particles (fluid / jelly / snow boxes) on the 1024-cell grid, one tick of 100 substeps
+  of the fused engine against the FP64 oracle (a transcription of MPM.cu:46-301) run on every particle.
+  Target of the workload: relative L2 position error <= 1e-4 after 100 substeps.
+- learned1024: the 4,194,304-particle jelly + snow scene with the committed closure weights
+  (mpm_amd/data/closure_learned1024.npy, the ones the benchmark loads), 100 substeps: learned against the
+  analytic engine and against the FP64 oracle within 1e-3.
+- learned16x2048: 16 scenes of 2048 cells, 91.5M particles, learned closure + batching: one full tick of
+  100 substeps keeps every particle finite and in the domain and every scene's grid mass at N_k m; the
+  smallest scene equals the FP64 oracle and the analytic engine run on it alone within 1e-3.
+
+Each test prints its measured errors before asserting, so the record keeps the numbers.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from mpm_amd import abi, scenes
+from mpm_amd.engine import Engine
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+WEIGHTS = os.path.join(ROOT, "gpu-physics-engine_amd", "mpm_amd", "data", "closure_learned1024.npy")
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    den = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (den if den > 0 else 1.0))
+
+
+def oracle_threads():
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def run_engine(params, scene, ticks=1, weights=None, fields=("x", "v", "F", "J"), grid=True):
+    with Engine(params) as e:
+        e.set_scene(scene)
+        if weights is not None:
+            e.set_closure(weights)
+        e.step(ticks)
+        e.sync()
+        out = e.readback(fields)
+        if grid:
+            out["grid_mass"] = e.read_grid()[0]
+    return out
+
+
+def test_mixed1024_full_size_against_oracle(oracle_mod):
+    """4,194,304 particles, one tick of 100 substeps of the fused engine (the benchmark's configuration)
+    against the FP64 oracle on every particle.  Bars: positions 1e-5 (ten times tighter than the workload's
+    1e-4 target), velocities 1e-3 (elastic oscillation amplifies FP32 rounding in v), F and J 1e-4, grid
+    mass of the last substep 1e-4."""
+    p = abi.preset(abi.PRESET_MIXED1024)
+    sc = scenes.mixed()
+    assert sc.n == 4194304
+    out = run_engine(p, sc)
+    s = oracle_mod.State(sc.x, sc.v, material=sc.material)
+    grid = oracle_mod.run(p, s, p.substeps, threads=oracle_threads(), want_grid=True)
+    errs = {"x": rel(out["x"], s.x), "v": rel(out["v"], s.v), "F": rel(out["F"], s.F), "J": rel(out["J"], s.J),
+            "grid_mass": rel(out["grid_mass"][0], grid[0])}
+    per_mat = {int(m): rel(out["x"][sc.material == m], s.x[sc.material == m]) for m in np.unique(sc.material)}
+    print(f"[mixed1024 full size, 100 substeps] {errs} position error per material {per_mat}", flush=True)
+    assert errs["x"] < 1e-5
+    assert errs["v"] < 1e-3
+    assert errs["F"] < 1e-4
+    assert errs["J"] < 1e-4
+    assert errs["grid_mass"] < 1e-4
+    assert all(e < 1e-5 for e in per_mat.values())
+
+
+def test_learned1024_full_size_committed_weights(oracle_mod):
+    """The benchmark's learned workload: 4,194,304 jelly + snow particles, the committed closure weights,
+    100 substeps.  Learned within 1e-3 of the analytic engine and of the FP64 oracle (the workload's bar);
+    the analytic engine within 1e-5 of the oracle."""
+    p = abi.preset(abi.PRESET_LEARNED1024)
+    sc = scenes.for_preset(abi.PRESET_LEARNED1024)
+    assert sc.n == 4194304
+    w = np.load(WEIGHTS)
+    learned = run_engine(p, sc, weights=w, fields=("x", "v"), grid=False)
+    pa = p.copy()
+    pa.closure = abi.CLOSURE_ANALYTIC
+    plain = run_engine(pa, sc, fields=("x", "v"), grid=False)
+    s = oracle_mod.State(sc.x, sc.v, material=sc.material)
+    oracle_mod.run(pa, s, p.substeps, threads=oracle_threads())
+    errs = {"learned_vs_plain": rel(learned["x"], plain["x"]), "learned_vs_oracle": rel(learned["x"], s.x),
+            "plain_vs_oracle": rel(plain["x"], s.x), "learned_v_vs_oracle": rel(learned["v"], s.v)}
+    print(f"[learned1024 full size, committed weights, 100 substeps] {errs}", flush=True)
+    assert np.isfinite(learned["x"]).all()
+    assert errs["plain_vs_oracle"] < 1e-5
+    assert errs["learned_vs_plain"] < 1e-3
+    assert errs["learned_vs_oracle"] < 1e-3
+
+
+def test_learned16x2048_full_size(oracle_mod):
+    """16 scenes of 2048 cells (91.5M particles, counts log-uniform in [1,048,576, 16,777,216]), learned
+    closure with the committed weights, one tick of 100 substeps in one handle: every particle finite and in
+    the domain, each scene's grid mass N_k m within 1e-4; the smallest scene (1,170,980 particles) equals
+    the FP64 oracle and the analytic engine run on it alone within 1e-3, and the analytic engine equals the
+    oracle within 1e-5."""
+    p = abi.preset(abi.PRESET_LEARNED16X2048)
+    sc = scenes.for_preset(abi.PRESET_LEARNED16X2048)
+    print(f"[learned16x2048] {sc.n} particles in {p.num_scenes} scenes", flush=True)
+    w = np.load(WEIGHTS)
+    out = run_engine(p, sc, weights=w, fields=("x",))
+    x = out["x"]
+    assert x.shape[0] == sc.n
+    assert np.isfinite(x).all()
+    assert x.min() >= 0.0 and x.max() <= 1.0
+    mass = out["grid_mass"].astype(np.float64).reshape(p.num_scenes, -1).sum(axis=1)
+    mass_err = np.abs(mass / (sc.scene_counts.astype(np.float64) * np.float64(np.float32(p.mass))) - 1.0)
+    k = int(np.argmin(sc.scene_counts))
+    sl = sc.scene_slices()[k]
+    xk = x[sl].copy()
+    del out, x
+    alone = scenes.Scene(sc.x[sl], sc.v[sl], sc.material[sl], [sl.stop - sl.start], sc.seed, "smallest scene")
+    del sc
+    pa = p.copy()
+    pa.num_scenes = 1
+    pa.capacity = alone.n
+    pa.closure = abi.CLOSURE_ANALYTIC
+    plain = run_engine(pa, alone, fields=("x",), grid=False)
+    s = oracle_mod.State(alone.x, alone.v, material=alone.material)
+    oracle_mod.run(pa, s, p.substeps, threads=min(8, oracle_threads()))
+    errs = {"scene": k, "n": alone.n, "max_mass_err": float(mass_err.max()),
+            "learned_vs_oracle": rel(xk, s.x), "learned_vs_plain": rel(xk, plain["x"]),
+            "plain_vs_oracle": rel(plain["x"], s.x)}
+    print(f"[learned16x2048 full size, 100 substeps] {errs}", flush=True)
+    assert mass_err.max() < 1e-4
+    assert errs["plain_vs_oracle"] < 1e-5
+    assert errs["learned_vs_plain"] < 1e-3
+    assert errs["learned_vs_oracle"] < 1e-3
