@@ -70,13 +70,15 @@ namespace {
 thread_local int32_t g_create_error = 0;
 thread_local std::string g_create_msg;
 
+// Records a failure: the message always, the runtime code (hipError_t, nonzero) only when one is given, as the
+// reference's record() keeps the runtime code for cuda_engine_last_error (cuda-render.cu:23-27).
 int32_t fail(MpmEngine* h, int32_t code, int32_t detail, const std::string& msg)
 {
     if (h) {
-        h->last_error = detail ? detail : code;
+        if (detail) { h->last_error = detail; }
         h->last_msg = msg;
     } else {
-        g_create_error = detail ? detail : code;
+        if (detail) { g_create_error = detail; }
         g_create_msg = msg;
     }
     return code;
@@ -725,7 +727,7 @@ int32_t mpm_create(const MpmParams* params, MpmEngine** out)
         return fail(nullptr, MPM_ERR_NO_DEVICE, 0, "no HIP device");
     }
     MpmEngine* h = new (std::nothrow) MpmEngine();
-    if (!h) { return fail(nullptr, MPM_ERR_INVALID_ARG, 0, "out of host memory"); }
+    if (!h) { return fail(nullptr, MPM_ERR_DEVICE, (int32_t)hipErrorOutOfMemory, "out of host memory"); }
     h->params = p;
     derive(p, h->dp);
     h->shape = shape_index(p.block_x, p.block_y);
@@ -737,13 +739,13 @@ int32_t mpm_create(const MpmParams* params, MpmEngine** out)
         delete h;
         return code;
     };
-    if (hipGetDevice(&h->device) != hipSuccess) { return bail(fail(h, MPM_ERR_DEVICE, 0, "hipGetDevice")); }
-    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
-        return bail(fail(h, MPM_ERR_DEVICE, 0, "hipStreamCreateWithFlags"));
-    }
-    if (hipEventCreate(&h->t0) != hipSuccess || hipEventCreate(&h->t1) != hipSuccess ||
-        hipEventCreateWithFlags(&h->done, hipEventDisableTiming) != hipSuccess) {
-        return bail(fail(h, MPM_ERR_DEVICE, 0, "hipEventCreate"));
+    hipError_t e = hipGetDevice(&h->device);
+    if (e != hipSuccess) { return bail(fail(h, MPM_ERR_DEVICE, (int32_t)e, "hipGetDevice")); }
+    e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { return bail(fail(h, MPM_ERR_DEVICE, (int32_t)e, "hipStreamCreateWithFlags")); }
+    if ((e = hipEventCreate(&h->t0)) != hipSuccess || (e = hipEventCreate(&h->t1)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&h->done, hipEventDisableTiming)) != hipSuccess) {
+        return bail(fail(h, MPM_ERR_DEVICE, (int32_t)e, "hipEventCreate"));
     }
     if ((rc = allocate(h)) != MPM_OK) { return bail(rc); }
     if ((rc = build_graphs(h)) != MPM_OK) { return bail(rc); }
@@ -790,9 +792,11 @@ int32_t mpm_step(MpmEngine* h, int32_t n_ticks)
     if (h->params.closure == MPM_CLOSURE_LEARNED && !h->closure_loaded) {
         return fail(h, MPM_ERR_STATE, 0, "learned closure selected but no weights loaded (mpm_set_closure)");
     }
+    if (n_ticks == 0) { return MPM_OK; }
+    // Every tick until the next publication writes the same slot: the write slot moves only in publish().
+    const int32_t slot = h->write_slot;
+    HIP_TRY(h, set_output_view_async(h->stream, &h->staged[slot]));
     for (int32_t k = 0; k < n_ticks; ++k) {
-        const int32_t slot = h->write_slot;
-        HIP_TRY(h, set_output_view_async(h->stream, &h->staged[slot]));
         const bool last = k == n_ticks - 1;
         const int bin = next_tick_bins(h) ? 1 : 0;
         if (last) { HIP_TRY(h, hipEventRecord(h->t0, h->stream)); }
@@ -800,14 +804,28 @@ int32_t mpm_step(MpmEngine* h, int32_t n_ticks)
         if (last) { HIP_TRY(h, hipEventRecord(h->t1, h->stream)); }
         h->cur ^= h->flips[bin];
         note_tick(h, bin);
-        h->slot_count[slot].store(h->last_count, std::memory_order_relaxed);
-        h->pending_slot = slot;
-        h->write_slot = (slot + 1) % 3;
         h->has_ticked = true;
     }
+    h->slot_count[slot].store(h->last_count, std::memory_order_relaxed);
+    h->pending_slot = slot;
     HIP_TRY(h, hipEventRecord(h->done, h->stream));
     return MPM_OK;
 }
+
+namespace {
+// OutputBuffer::roll (cuda-engine.cpp:72-76), called once the ticks writing the pending slot have finished:
+// that slot becomes the read slot and the write slot moves to the one after it.  Slots then go round in
+// order, so the new write slot is neither the read slot nor the slot read before it (the cushion for a
+// reader that took the previous read slot just before this store).
+void publish(MpmEngine* h)
+{
+    if (h->pending_slot < 0) { return; }
+    const int32_t s = h->pending_slot;
+    h->read_slot.store(s, std::memory_order_release);
+    h->write_slot = (s + 1) % 3;
+    h->pending_slot = -1;
+}
+}  // namespace
 
 int32_t mpm_query(MpmEngine* h)
 {
@@ -815,7 +833,7 @@ int32_t mpm_query(MpmEngine* h)
     const hipError_t e = hipEventQuery(h->done);
     if (e == hipErrorNotReady) { return MPM_BUSY; }
     if (e != hipSuccess) { return fail(h, MPM_ERR_DEVICE, (int32_t)e, "hipEventQuery"); }
-    if (h->pending_slot >= 0) { h->read_slot.store(h->pending_slot, std::memory_order_release); }
+    publish(h);
     return MPM_OK;
 }
 
@@ -823,7 +841,7 @@ int32_t mpm_sync(MpmEngine* h)
 {
     if (!h) { return MPM_ERR_INVALID_ARG; }
     HIP_TRY(h, hipStreamSynchronize(h->stream));
-    if (h->pending_slot >= 0) { h->read_slot.store(h->pending_slot, std::memory_order_release); }
+    publish(h);
     return MPM_OK;
 }
 
@@ -1112,18 +1130,21 @@ int32_t mpm_autotune(const MpmParams* base, int32_t n, const float* x, const flo
 
 int32_t mpm_last_error(MpmEngine* h) { return h ? h->last_error : g_create_error; }
 
-const char* mpm_error_name(int32_t err)
+const char* mpm_error_name(int32_t err) { return hipGetErrorName((hipError_t)err); }
+
+const char* mpm_status_name(int32_t status)
 {
-    switch (err) {
+    switch (status) {
     case MPM_OK: return "MPM_OK";
     case MPM_BUSY: return "MPM_BUSY";
-    case MPM_ERR_INVALID_ARG: return "MPM_ERR_INVALID_ARG";
     case MPM_ERR_DEVICE: return "MPM_ERR_DEVICE";
-    case MPM_ERR_CAPACITY: return "MPM_ERR_CAPACITY";
     case MPM_ERR_NO_DEVICE: return "MPM_ERR_NO_DEVICE";
     case MPM_ERR_STATE: return "MPM_ERR_STATE";
+    case MPM_ERR_INVALID_ARG: return "MPM_ERR_INVALID_ARG";
+    case MPM_ERR_ALREADY: return "MPM_ERR_ALREADY";
+    case MPM_ERR_CAPACITY: return "MPM_ERR_CAPACITY";
     case MPM_ERR_NOT_BUILT: return "MPM_ERR_NOT_BUILT";
-    default: return hipGetErrorName((hipError_t)err);
+    default: return "MPM_UNKNOWN_STATUS";
     }
 }
 

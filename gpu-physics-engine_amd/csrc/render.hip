@@ -96,9 +96,10 @@ __global__ void k_render_compose(RenderArgs a, BlendLut lut, const uint32_t* cov
 thread_local int32_t g_render_error = 0;
 thread_local std::string g_render_msg;
 
+// The runtime code (hipError_t) is kept only when one is given, like record() in cuda-render.cu:23-27.
 int32_t render_fail(int32_t code, int32_t detail, const std::string& msg)
 {
-    g_render_error = detail ? detail : code;
+    if (detail) { g_render_error = detail; }
     g_render_msg = msg;
     return code;
 }

@@ -88,7 +88,7 @@ Options parse(int argc, char** argv)
 void check(MpmEngine* h, int32_t rc, const char* what)
 {
     if (rc < 0) {
-        std::fprintf(stderr, "mpm_run: %s failed: %s (%d): %s\n", what, mpm_error_name(rc), rc, mpm_last_message(h));
+        std::fprintf(stderr, "mpm_run: %s failed: %s (%d): %s\n", what, mpm_status_name(rc), rc, mpm_last_message(h));
         std::exit(1);
     }
 }
@@ -96,7 +96,7 @@ void check(MpmEngine* h, int32_t rc, const char* what)
 void check_render(int32_t rc, const char* what)
 {
     if (rc < 0) {
-        std::fprintf(stderr, "mpm_run: %s failed: %s (%d): %s\n", what, mpm_error_name(rc), rc, mpm_render_message());
+        std::fprintf(stderr, "mpm_run: %s failed: %s (%d): %s\n", what, mpm_status_name(rc), rc, mpm_render_message());
         std::exit(1);
     }
 }

@@ -12,12 +12,13 @@ LIB_PATH = os.path.join(_PKG_DIR, "lib", "libmpm_engine.so")
 
 MPM_OK = 0
 MPM_BUSY = 1
-MPM_ERR_INVALID_ARG = -1
-MPM_ERR_DEVICE = -2
-MPM_ERR_CAPACITY = -3
-MPM_ERR_NO_DEVICE = -4
-MPM_ERR_STATE = -5
-MPM_ERR_NOT_BUILT = -6
+MPM_ERR_DEVICE = -1
+MPM_ERR_NO_DEVICE = -2
+MPM_ERR_STATE = -3
+MPM_ERR_INVALID_ARG = -4
+MPM_ERR_ALREADY = -5
+MPM_ERR_CAPACITY = -6
+MPM_ERR_NOT_BUILT = -7
 
 MAT_FLUID = 0
 MAT_JELLY = 1
@@ -161,6 +162,7 @@ SIGNATURES = {
     "mpm_set_closure": (ctypes.c_int32, [_vp, _f32p]),
     "mpm_last_error": (ctypes.c_int32, [_vp]),
     "mpm_error_name": (ctypes.c_char_p, [ctypes.c_int32]),
+    "mpm_status_name": (ctypes.c_char_p, [ctypes.c_int32]),
     "mpm_last_message": (ctypes.c_char_p, [_vp]),
     # include/mpm_render.h
     "mpm_renderer_create": (ctypes.c_int32, [_P(MpmRenderDesc), _P(_vp)]),

@@ -21,15 +21,24 @@ from . import abi
 
 
 class MpmError(RuntimeError):
-    def __init__(self, code, message):
-        super().__init__(f"{abi.load_library().mpm_error_name(code).decode()} ({code}): {message}")
+    """A negative status of the C-ABI; `code` is the status, `runtime_error` the hipError_t that
+    mpm_last_error holds when the status is MPM_ERR_DEVICE (else None)."""
+
+    def __init__(self, code, message, runtime_error=None):
+        lib = abi.load_library()
+        text = f"{lib.mpm_status_name(code).decode()} ({code}): {message}"
+        if runtime_error is not None:
+            text += f" [{lib.mpm_error_name(runtime_error).decode()} ({runtime_error})]"
+        super().__init__(text)
         self.code = code
+        self.runtime_error = runtime_error
 
 
 def _check(lib, h, rc, what):
     if rc < 0:
         msg = lib.mpm_last_message(h)
-        raise MpmError(rc, f"{what}: {msg.decode() if msg else ''}")
+        rt = lib.mpm_last_error(h) if rc == abi.MPM_ERR_DEVICE else None
+        raise MpmError(rc, f"{what}: {msg.decode() if msg else ''}", rt)
     return rc
 
 
@@ -47,7 +56,8 @@ class Engine:
         rc = self.lib.mpm_create(ctypes.byref(self.params), ctypes.byref(h))
         if rc < 0:
             msg = self.lib.mpm_last_message(None)
-            raise MpmError(rc, f"mpm_create: {msg.decode() if msg else ''}")
+            rt = self.lib.mpm_last_error(None) if rc == abi.MPM_ERR_DEVICE else None
+            raise MpmError(rc, f"mpm_create: {msg.decode() if msg else ''}", rt)
         self.h = h
         self.n = 0
 

@@ -13,7 +13,7 @@ from . import abi
 
 class RenderError(RuntimeError):
     def __init__(self, code, message):
-        super().__init__(f"{abi.load_library().mpm_error_name(code).decode()} ({code}): {message}")
+        super().__init__(f"{abi.load_library().mpm_status_name(code).decode()} ({code}): {message}")
         self.code = code
 
 

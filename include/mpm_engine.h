@@ -35,15 +35,24 @@
 extern "C" {
 #endif
 
-/* ---- status codes (reference render C-ABI uses 0 / -1 / -2 / -4 / -5, cuda-render.cu:23-60) ---- */
+/* ---- status codes ----
+ * -1 .. -5 keep the meanings of the reference's only C-ABI (libs/render/source/cuda-render.cu):
+ *   -1 a runtime call failed, its code in last_error (record(), :23-27);  -2 no matching device (:46);
+ *   -3 not initialised / not bound (:80, :171);  -4 invalid argument (:31, :81, :89);
+ *   -5 already initialised / bound (:32, :82-84).
+ * -6 and -7 have no counterpart there.  MPM_BUSY is mpm_query's "still running" (cudaErrorNotReady,
+ * main/MPM-master.cpp:83-84). */
 #define MPM_OK 0
-#define MPM_BUSY 1                  /* mpm_query: work still in flight (cudaErrorNotReady, MPM-master.cpp:83-84) */
-#define MPM_ERR_INVALID_ARG (-1)    /* null handle/pointer, bad size, parameter out of range               */
-#define MPM_ERR_DEVICE (-2)         /* a HIP runtime call failed; mpm_last_error holds the hipError_t       */
-#define MPM_ERR_CAPACITY (-3)       /* particle count above the handle's capacity                            */
-#define MPM_ERR_NO_DEVICE (-4)      /* no usable gfx950 device                                               */
-#define MPM_ERR_STATE (-5)          /* call not valid in the handle's current state                          */
-#define MPM_ERR_NOT_BUILT (-6)      /* requested code path is not compiled into this library                 */
+#define MPM_BUSY 1                  /* mpm_query: work still in flight                                      */
+#define MPM_ERR_DEVICE (-1)         /* a HIP runtime call failed; mpm_last_error holds the hipError_t       */
+#define MPM_ERR_NO_DEVICE (-2)      /* no usable gfx950 device                                               */
+#define MPM_ERR_STATE (-3)          /* not ready: the call needs a state the handle is not in (no weights
+                                       loaded, no tick run yet, no binning in reference mode)              */
+#define MPM_ERR_INVALID_ARG (-4)    /* null handle/pointer, bad size, parameter out of range               */
+#define MPM_ERR_ALREADY (-5)        /* reserved with the reference's meaning ("already initialised"); handles
+                                       are created, never initialised twice, so no call returns it now      */
+#define MPM_ERR_CAPACITY (-6)       /* particle count above the handle's capacity                            */
+#define MPM_ERR_NOT_BUILT (-7)      /* requested code path is not compiled into this library                 */
 
 /* ---- materials (per particle, uint8) ---- */
 #define MPM_MAT_FLUID 0 /* Tait equation of state on the tracked J (reference MPM.cu:68-80)             */
@@ -132,17 +141,25 @@ int32_t mpm_set_particles(MpmEngine* h, int32_t n, const float* x, const float* 
                           const float* F, const float* J, const uint8_t* material, const int32_t* scene_counts);
 
 /* Enqueue n_ticks ticks of params.substeps substeps each, asynchronously (one graph launch per tick,
- * MPM-master.cpp:106-109).  Each tick ends with the output copy into the next output slot. */
+ * MPM-master.cpp:106-109).  Each tick ends with the output copy (MPMOutputKernel, MPM.cu:302-322) into the
+ * write slot of the output triple buffer.  The write slot moves only when a finished tick is published
+ * (mpm_query / mpm_sync, the reference's roll after tick_done, MPM-master.cpp:83-89), so every tick enqueued
+ * between two publications writes the same slot, and no tick ever writes the published read slot or the
+ * slot published before it (the cushion of cuda-engine.h:59-61). */
 int32_t mpm_step(MpmEngine* h, int32_t n_ticks);
-/* MPM_OK when all enqueued work finished, MPM_BUSY while in flight (cudaEventQuery, MPM-master.cpp:83-84). */
+/* MPM_OK when all enqueued work finished, MPM_BUSY while in flight (cudaEventQuery, MPM-master.cpp:83-84).
+ * On MPM_OK a tick written since the last publication is published: it becomes the read slot, and the
+ * write slot moves to the slot after it (OutputBuffer::roll, cuda-engine.cpp:72-76). */
 int32_t mpm_query(MpmEngine* h);
-/* Block until all enqueued work finished. */
+/* Block until all enqueued work finished, then publish like mpm_query. */
 int32_t mpm_sync(MpmEngine* h);
 /* GPU time of the last tick in ms (cudaEventElapsedTime(t0, t1), MPM-master.cpp:90-92). */
 int32_t mpm_last_step_ms(MpmEngine* h, float* ms);
 
-/* Device pointers of the slot most recently completed by a tick (OutputBuffer::readTarget, cuda-engine.cpp:78-87).
- * Safe to call from a second thread. */
+/* Device pointers of the slot most recently published (OutputBuffer::readTarget, cuda-engine.cpp:78-87).
+ * Safe to call from a second thread.  The slot's contents stay unchanged until two more publications:
+ * the next one moves the read slot on, and only the one after it lets a tick write this slot again, so a
+ * reader has one publication period to finish with a view (the reference's third slot). */
 int32_t mpm_read_view(MpmEngine* h, MpmView* out);
 
 /* Synchronous host copy of the particle state in the caller's particle order.  Any pointer may be NULL.
@@ -222,11 +239,16 @@ typedef struct MpmTuneResult {
 int32_t mpm_autotune(const MpmParams* base, int32_t n, const float* x, const float* v, const uint8_t* material,
                      const int32_t* scene_counts, int32_t substeps, MpmTuneResult* out);
 
-/* Last hipError_t (or MPM_ERR_*) recorded on the handle (cuda_engine_last_error, cuda-interface.h:41);
- * h may be NULL for errors raised before a handle existed. */
+/* hipError_t of the last runtime call that failed on the handle, 0 if none (cuda_engine_last_error,
+ * cuda-interface.h:41, cuda-render.cu:158); h may be NULL for failures before a handle existed (mpm_create,
+ * mpm_autotune).  A status other than MPM_ERR_DEVICE leaves it unchanged. */
 int32_t mpm_last_error(MpmEngine* h);
-/* Static name of a status code (cuda_engine_error_name, cuda-interface.h:42). */
+/* Name of a runtime error code as returned by mpm_last_error (cuda_engine_error_name: cudaGetErrorName,
+ * cuda-render.cu:159). */
 const char* mpm_error_name(int32_t err);
+/* Static name of an MPM_* status code ("MPM_ERR_INVALID_ARG", ...; "MPM_UNKNOWN_STATUS" otherwise).  No
+ * counterpart in the reference, whose statuses are bare integers. */
+const char* mpm_status_name(int32_t status);
 /* Human readable message of the last error on the handle (NULL handle: last create failure). */
 const char* mpm_last_message(MpmEngine* h);
 

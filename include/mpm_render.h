@@ -54,8 +54,8 @@ int32_t mpm_render_frame(MpmRenderer* r, const MpmView* view, uint8_t* rgba, int
 /* Write an RGBA8 host image as a binary PPM (P6, alpha dropped).  Host only. */
 int32_t mpm_write_ppm(const char* path, const uint8_t* rgba, int32_t width, int32_t height, int32_t row_pitch);
 
-/* Last error of a render call on this thread (hipError_t or MPM_ERR_*) and its message
- * (cuda_engine_last_error, cuda-render.cu:158). */
+/* hipError_t of the last runtime call that failed in a render call on this thread, 0 if none
+ * (cuda_engine_last_error, cuda-render.cu:158), and the message of the last failed render call. */
 int32_t mpm_render_last_error(void);
 const char* mpm_render_message(void);
 

@@ -102,8 +102,37 @@ def test_null_handle_and_error_names():
     assert lib.mpm_step(None, 1) == abi.MPM_ERR_INVALID_ARG
     assert lib.mpm_sync(None) == abi.MPM_ERR_INVALID_ARG
     assert lib.mpm_destroy(None) == abi.MPM_ERR_INVALID_ARG
-    assert lib.mpm_error_name(abi.MPM_ERR_CAPACITY) == b"MPM_ERR_CAPACITY"
-    assert lib.mpm_error_name(abi.MPM_OK) == b"MPM_OK"
+    assert lib.mpm_status_name(abi.MPM_ERR_CAPACITY) == b"MPM_ERR_CAPACITY"
+    assert lib.mpm_status_name(abi.MPM_OK) == b"MPM_OK"
+    assert lib.mpm_status_name(-99) == b"MPM_UNKNOWN_STATUS"
+    # mpm_error_name names runtime codes, like cuda_engine_error_name (cudaGetErrorName, cuda-render.cu:159)
+    assert lib.mpm_error_name(0) == b"hipSuccess"
+    assert lib.mpm_error_name(2) == b"hipErrorOutOfMemory"
+
+
+def test_status_codes_keep_the_reference_meanings():
+    """-1 .. -5 mean what the reference's C-ABI returns for them (libs/render/source/cuda-render.cu):
+    record() turns a failed runtime call into -1 (:23-27), no matching adapter is -2 (:46), use before
+    init / bind is -3 (:80, :171), a bad argument is -4 (:31, :81, :89), init / bind twice is -5 (:32, :82)."""
+    import re
+    hdr = open(os.path.join(ROOT, "include", "mpm_engine.h")).read()
+    defs = {m.group(1): int(m.group(2)) for m in re.finditer(r"#define (MPM_[A-Z_]+) \(?(-?\d+)\)?", hdr)}
+    assert defs["MPM_ERR_DEVICE"] == -1 == abi.MPM_ERR_DEVICE
+    assert defs["MPM_ERR_NO_DEVICE"] == -2 == abi.MPM_ERR_NO_DEVICE
+    assert defs["MPM_ERR_STATE"] == -3 == abi.MPM_ERR_STATE
+    assert defs["MPM_ERR_INVALID_ARG"] == -4 == abi.MPM_ERR_INVALID_ARG
+    assert defs["MPM_ERR_ALREADY"] == -5 == abi.MPM_ERR_ALREADY
+    assert defs["MPM_ERR_CAPACITY"] == abi.MPM_ERR_CAPACITY and defs["MPM_ERR_NOT_BUILT"] == abi.MPM_ERR_NOT_BUILT
+    statuses = [v for k, v in defs.items() if k.startswith("MPM_ERR_") or k in ("MPM_OK", "MPM_BUSY")]
+    assert len(statuses) == len(set(statuses))
+    # no handle, failure before any runtime call: the runtime code stays as it was (0: none failed yet)
+    lib = abi.load_library()
+    bad = abi.preset(abi.PRESET_JELLY128)
+    bad.halo = 1
+    h = ctypes.c_void_p()
+    before = lib.mpm_last_error(None)
+    assert lib.mpm_create(ctypes.byref(bad), ctypes.byref(h)) == abi.MPM_ERR_INVALID_ARG
+    assert lib.mpm_last_error(None) == before
 
 
 def test_engine_without_device_fails_loudly():
