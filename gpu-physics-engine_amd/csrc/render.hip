@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <mutex>
 #include <string>
 
 #include "../../include/mpm_render.h"
@@ -131,6 +132,85 @@ BlendLut make_lut()
     return l;
 }
 
+// Checks a description; *pitch receives the row pitch in bytes.
+int32_t check_desc(const MpmRenderDesc& d, int* pitch)
+{
+    if (d.width < 1 || d.height < 1 || d.width > 32768 || d.height > 32768) {
+        return render_fail(MPM_ERR_INVALID_ARG, 0, "image size outside [1, 32768]");
+    }
+    *pitch = d.row_pitch ? d.row_pitch : 4 * d.width;
+    if (*pitch < 4 * d.width || *pitch % 4 != 0) {
+        return render_fail(MPM_ERR_INVALID_ARG, 0, "row_pitch below 4 * width or not a multiple of 4");
+    }
+    if (!(d.scale > 0.f) || !std::isfinite(d.scale) || !(d.flip_extent >= 0.f) || !std::isfinite(d.flip_extent) ||
+        d.chunk_px < 0) {
+        return render_fail(MPM_ERR_INVALID_ARG, 0, "scale must be > 0, flip_extent >= 0, chunk_px >= 0");
+    }
+    return MPM_OK;
+}
+
+int32_t check_view(const MpmView* view)
+{
+    if (view->count < 0 || (view->count > 0 && !view->pos)) {
+        return render_fail(MPM_ERR_INVALID_ARG, 0, "view has a negative count or no positions");
+    }
+    return MPM_OK;
+}
+
+// The three passes of a frame on one stream: clear the counters of the image's pixels, count coverage,
+// compose into target (device memory).
+hipError_t enqueue_frame(hipStream_t st, const RenderArgs& a, const BlendLut& lut, uint32_t* cov, const MpmView& view,
+                         uint8_t* target)
+{
+    const size_t px = (size_t)a.w * a.h;
+    hipLaunchKernelGGL(k_render_clear, dim3((unsigned)std::min<size_t>((px + kThreads - 1) / kThreads, 65536)),
+                       dim3(kThreads), 0, st, cov, px);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) { return e; }
+    if (view.count > 0) {
+        hipLaunchKernelGGL(k_render_cover, dim3((unsigned)((view.count + kThreads - 1) / kThreads)), dim3(kThreads), 0,
+                           st, a, view.pos, view.count, cov);
+        if ((e = hipGetLastError()) != hipSuccess) { return e; }
+    }
+    hipLaunchKernelGGL(k_render_compose, dim3((unsigned)((a.w + kThreads - 1) / kThreads), (unsigned)a.h),
+                       dim3(kThreads), 0, st, a, lut, cov, target);
+    return hipGetLastError();
+}
+
+// ---- the no-allocation frame (mpm_render_frame_noalloc) ----
+// Coverage counters of mpm_render_frame_noalloc: one per pixel, for images of at most kScratchPixels pixels
+// (8 MiB per device, held by every device the library runs on until the process ends).  Host code never
+// uses the array's name as an address: scratch_on_device() looks the device address up once per device.
+constexpr size_t kScratchPixels = (size_t)2048 * 1024;
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+uint32_t* g_scratch_ptr[kScratchDevices] = {};
+
+}  // namespace
+
+static __device__ __align__(256) uint32_t scratch[kScratchPixels];
+
+namespace {
+
+// Device address of `scratch` on device `dev`, which must be the current device (hipGetSymbolAddress
+// answers for the current device; each device holds its own copy of the array).
+hipError_t scratch_on_device(int dev, uint32_t** out)
+{
+    if (dev < 0 || dev >= kScratchDevices) { return hipErrorInvalidDevice; }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) { return e; }
+        if (!p) { return hipErrorInvalidSymbol; }
+        g_scratch_ptr[dev] = static_cast<uint32_t*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
+
+const BlendLut g_lut = make_lut();
+
 }  // namespace
 
 struct MpmRenderer {
@@ -150,17 +230,9 @@ int32_t mpm_renderer_create(const MpmRenderDesc* desc, MpmRenderer** out)
     if (!desc || !out) { return render_fail(MPM_ERR_INVALID_ARG, 0, "desc or out is NULL"); }
     *out = nullptr;
     const MpmRenderDesc& d = *desc;
-    if (d.width < 1 || d.height < 1 || d.width > 32768 || d.height > 32768) {
-        return render_fail(MPM_ERR_INVALID_ARG, 0, "image size outside [1, 32768]");
-    }
-    const int pitch = d.row_pitch ? d.row_pitch : 4 * d.width;
-    if (pitch < 4 * d.width || pitch % 4 != 0) {
-        return render_fail(MPM_ERR_INVALID_ARG, 0, "row_pitch below 4 * width or not a multiple of 4");
-    }
-    if (!(d.scale > 0.f) || !std::isfinite(d.scale) || !(d.flip_extent >= 0.f) || !std::isfinite(d.flip_extent) ||
-        d.chunk_px < 0) {
-        return render_fail(MPM_ERR_INVALID_ARG, 0, "scale must be > 0, flip_extent >= 0, chunk_px >= 0");
-    }
+    int pitch = 0;
+    const int32_t bad = check_desc(d, &pitch);
+    if (bad != MPM_OK) { return bad; }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
         return render_fail(MPM_ERR_NO_DEVICE, 0, "no HIP device");
@@ -195,29 +267,39 @@ int32_t mpm_renderer_destroy(MpmRenderer* r)
 int32_t mpm_render_frame(MpmRenderer* r, const MpmView* view, uint8_t* rgba, int32_t rgba_on_device)
 {
     if (!r || !view || !rgba) { return render_fail(MPM_ERR_INVALID_ARG, 0, "renderer, view or rgba is NULL"); }
-    if (view->count < 0 || (view->count > 0 && !view->pos)) {
-        return render_fail(MPM_ERR_INVALID_ARG, 0, "view has a negative count or no positions");
-    }
+    if (check_view(view) != MPM_OK) { return MPM_ERR_INVALID_ARG; }
     const MpmRenderDesc& d = r->d;
     RenderArgs a{d.width, d.height, r->pitch, d.chunk_px, d.scale, d.flip_extent};
     uint8_t* target = rgba_on_device ? rgba : r->img;
-    const size_t px = (size_t)d.width * d.height;
     RENDER_TRY(hipSetDevice(r->device));
-    hipLaunchKernelGGL(k_render_clear, dim3((unsigned)std::min<size_t>((px + kThreads - 1) / kThreads, 65536)),
-                       dim3(kThreads), 0, r->stream, r->cov, px);
-    RENDER_TRY(hipGetLastError());
-    if (view->count > 0) {
-        hipLaunchKernelGGL(k_render_cover, dim3((unsigned)((view->count + kThreads - 1) / kThreads)), dim3(kThreads), 0,
-                           r->stream, a, view->pos, view->count, r->cov);
-        RENDER_TRY(hipGetLastError());
-    }
-    hipLaunchKernelGGL(k_render_compose, dim3((unsigned)((d.width + kThreads - 1) / kThreads), (unsigned)d.height),
-                       dim3(kThreads), 0, r->stream, a, r->lut, r->cov, target);
-    RENDER_TRY(hipGetLastError());
+    RENDER_TRY(enqueue_frame(r->stream, a, r->lut, r->cov, *view, target));
     if (!rgba_on_device) {
         RENDER_TRY(hipMemcpyAsync(rgba, r->img, (size_t)r->pitch * d.height, hipMemcpyDeviceToHost, r->stream));
     }
     RENDER_TRY(hipStreamSynchronize(r->stream));
+    return MPM_OK;
+}
+
+int32_t mpm_render_frame_noalloc(const MpmRenderDesc* desc, const MpmView* view, uint8_t* rgba_device)
+{
+    if (!desc || !view || !rgba_device) { return render_fail(MPM_ERR_INVALID_ARG, 0, "desc, view or rgba is NULL"); }
+    int pitch = 0;
+    const int32_t bad = check_desc(*desc, &pitch);
+    if (bad != MPM_OK) { return bad; }
+    if (check_view(view) != MPM_OK) { return MPM_ERR_INVALID_ARG; }
+    if ((size_t)desc->width * (size_t)desc->height > kScratchPixels) {
+        return render_fail(MPM_ERR_CAPACITY, 0,
+                           "image of " + std::to_string(desc->width) + " x " + std::to_string(desc->height) +
+                               " pixels exceeds the coverage scratch of " + std::to_string(kScratchPixels) +
+                               " pixels of mpm_render_frame_noalloc");
+    }
+    int dev = 0;
+    RENDER_TRY(hipGetDevice(&dev));
+    uint32_t* cov = nullptr;
+    RENDER_TRY(scratch_on_device(dev, &cov));
+    const RenderArgs a{desc->width, desc->height, pitch, desc->chunk_px, desc->scale, desc->flip_extent};
+    RENDER_TRY(enqueue_frame(nullptr, a, g_lut, cov, *view, rgba_device));
+    RENDER_TRY(hipStreamSynchronize(nullptr));
     return MPM_OK;
 }
 

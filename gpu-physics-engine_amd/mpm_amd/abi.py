@@ -168,6 +168,7 @@ SIGNATURES = {
     "mpm_renderer_create": (ctypes.c_int32, [_P(MpmRenderDesc), _P(_vp)]),
     "mpm_renderer_destroy": (ctypes.c_int32, [_vp]),
     "mpm_render_frame": (ctypes.c_int32, [_vp, _P(MpmView), _vp, ctypes.c_int32]),
+    "mpm_render_frame_noalloc": (ctypes.c_int32, [_P(MpmRenderDesc), _P(MpmView), _vp]),
     "mpm_write_ppm": (ctypes.c_int32, [ctypes.c_char_p, _u8p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
     "mpm_render_last_error": (ctypes.c_int32, []),
     "mpm_render_message": (ctypes.c_char_p, []),

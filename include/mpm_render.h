@@ -51,6 +51,17 @@ int32_t mpm_renderer_destroy(MpmRenderer* r);
  * cudaStreamSynchronize after launchFill (cuda-render.cu:179). */
 int32_t mpm_render_frame(MpmRenderer* r, const MpmView* view, uint8_t* rgba, int32_t rgba_on_device);
 
+/* No counterpart in the reference: mpm_render_frame for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  Draws the frame of *desc
+ * (validated like mpm_renderer_create) into rgba_device, a device image of desc->row_pitch bytes per row,
+ * with no renderer and no allocation: the per-pixel coverage counters live in a device array of the
+ * library, 2048 x 1024 = 2,097,152 counters (8 MiB) per device, held by each device the library runs on
+ * until the process ends.  An image of more pixels returns MPM_ERR_CAPACITY.  Runs on the current device's
+ * null stream and returns when the frame is done.  Every call on one device shares that device's
+ * counters: calls that overlap on one device (from several host threads) race and must be serialised by
+ * the caller. */
+int32_t mpm_render_frame_noalloc(const MpmRenderDesc* desc, const MpmView* view, uint8_t* rgba_device);
+
 /* Write an RGBA8 host image as a binary PPM (P6, alpha dropped).  Host only. */
 int32_t mpm_write_ppm(const char* path, const uint8_t* rgba, int32_t width, int32_t height, int32_t row_pitch);
 
