@@ -2,11 +2,13 @@
 
 Workload (default): the 1024-cell mixed scene — 4,194,304 particles in three seeded boxes (fluid bulk 400,
 fixed-corotated jelly E = 1e4 nu = 0.2, snow theta_c = 2.5e-2 theta_s = 7.5e-3 hardening 10), initial
-velocity uniform in [-1,1]^2, dt = 2e-5.  One step = one tick of 100 substeps (the workload's run length),
-launched as one hipGraph; particle state is resident in HBM before the timed region.
+velocity uniform in [-1,1]^2, dt = 2e-5, particle density 16.  One step = one tick of 100 substeps (the
+workload's run length), launched as one hipGraph; particle state is resident in HBM before the timed region.
 
-Prints one JSON line (rank 0): throughput, per-step time, the roofline of the dominant kernel measured
-with HIP events on the engine's stream, and the FP64 oracle timed on this host's cores (bounded sample).
+Prints one JSON line: throughput, per-step time, the roofline of the dominant kernel timed with HIP events on
+the engine's stream, the FP64 oracle timed on this host's cores (bounded sample), and two more workloads
+timed the same way: the 64-scene batch (`batched`) and the learned closure on the 4,194,304-particle jelly +
+snow scene against the analytic engine on the same scene (`learned`).
 """
 import argparse
 import json
@@ -36,6 +38,9 @@ G2P2G_BYTES_PER_NODE = 36
 # Whole-substep algorithmic bytes of the unfused three-kernel form: 132 B per particle, 40 B per node.
 SUBSTEP_BYTES_PER_PARTICLE = 132
 SUBSTEP_BYTES_PER_NODE = 40
+# Kernel time per step (events, a replay of the timed window) may exceed the timed step by this much (run to
+# run spread of the atomic-order noise and of the clocks) before the bench refuses its own roofline.
+CONSISTENCY_SLACK = 1.02
 
 
 def binning_bytes_per_particle(p):
@@ -69,19 +74,34 @@ CLOSURE_WEIGHTS = os.path.join(ROOT, "gpu-physics-engine_amd", "mpm_amd", "data"
 
 
 def traffic_from_profile(p, workload):
-    """HBM bytes per launch of k_g2p2g measured by rocprofv3 PMC for this workload and launch shape
-    (profiles/g2p2g_<workload>_traffic.json), or None."""
-    path = os.path.join(ROOT, "profiles", f"g2p2g_{workload}_traffic.json")
+    """HBM bytes per launch of k_g2p2g measured by rocprofv3 PMC (FETCH_SIZE, WRITE_SIZE, each scaled by the
+    same counter on a device copy of known size) for this workload and launch shape, as committed in
+    profiles/g2p2g_<workload>_traffic.json; (bytes, source) or (None, None)."""
+    name = f"g2p2g_{workload}_traffic.json"
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
-        return None
+        return None, None
     d = json.load(open(path))
     if list(d.get("tile", [])) != [p.block_x, p.block_y] or d.get("chunk") != p.chunk:
-        return None
-    return d["traffic_bytes"]
+        return None, None
+    return d["traffic_bytes"], f"profiles/{name} (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; replayed, not measured here)"
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def scene_layout(workload, scene, p):
+    """What the synthetic scene is: density, seed, boxes (or per-scene counts)."""
+    rho = float(p.mass) / float(p.vol)
+    out = {"rho": round(rho, 6), "seed": int(scene.seed), "dt": float(np.float32(p.dt))}
+    if workload in ("mixed1024", "learned1024"):
+        out["boxes"] = [[list(lo), list(hi)] for lo, hi in scenes.MIXED_BOXES]
+        out["box_materials"] = ["fluid", "jelly", "snow"] if workload == "mixed1024" else ["jelly", "jelly", "snow"]
+        out["v0"] = "uniform in [-1,1]^2"
+    elif p.num_scenes > 1:
+        out["scene_counts_min_max"] = [int(scene.scene_counts.min()), int(scene.scene_counts.max())]
+    return out
 
 
 def cpu_baseline(params, scene, budget_s=12.0):
@@ -133,6 +153,148 @@ def copy_calibration(mib, reps=5):
     log(f"copy calibration: {reps} device copies of {mib} MiB")
 
 
+def timed_run(p, scene, steps, warmup, weights=None, kernel_timing=True, label=""):
+    """Warm-up ticks, then `steps` ticks timed by the host clock around mpm_step + mpm_sync (the engine's
+    stream is idle before and after).  With kernel_timing, a second engine from the same initial state runs
+    the same warm-up ticks and then the same window of ticks through mpm_time_kernels (each run of consecutive
+    launches of one class captured into a graph and launched between HIP events on the engine's stream), so
+    the per-class kernel times belong to exactly the ticks that were timed (the scene compresses under gravity
+    and the fused kernel slows as it does).  Returns (result dict, readback of x after the timed ticks)."""
+    def engine():
+        e = Engine(p)
+        e.set_scene(scene)
+        if weights is not None:
+            e.set_closure(weights)
+        for _ in range(warmup):
+            e.step(1)
+        e.sync()
+        return e
+
+    with engine() as eng:
+        t0 = time.perf_counter()
+        eng.step(steps)
+        eng.sync()
+        wall = time.perf_counter() - t0
+        last_tick_ms = eng.last_step_ms()
+        out = eng.readback(("x",))
+    ms_per_step = wall * 1e3 / steps
+    value = scene.n * p.substeps * steps / wall
+    log(f"{label}timed {steps} ticks: {ms_per_step:.3f} ms/tick (last tick by HIP events {last_tick_ms:.3f} ms), "
+        f"{value:.4g} particle-substeps/s")
+    res = {"value": value, "ms_per_step": ms_per_step, "steps": steps, "warmup": warmup,
+           "last_tick_ms_events": last_tick_ms}
+    if kernel_timing:
+        with engine() as twin:
+            res["kt"] = twin.time_kernels(steps * p.substeps)
+        res["kt_substeps"] = steps * p.substeps
+    return res, out["x"]
+
+
+def roofline_of(p, scene, run, workload):
+    """Roofline object of the dominant kernel from the graph-timed kernel classes of a timed run."""
+    kt = run["kt"]
+    nodes = p.grid_w * p.grid_h * p.num_scenes
+    dominant = "g2p2g" if p.mode == abi.MODE_FUSED else ("p2g" if p.mode == abi.MODE_REFERENCE else "g2p")
+    ms_dom, n_dom = kt[dominant]
+    if dominant == "g2p2g":
+        bytes_per_launch = scene.n * G2P2G_BYTES_PER_PARTICLE + nodes * G2P2G_BYTES_PER_NODE
+    elif dominant == "p2g":
+        bytes_per_launch = scene.n * 53 + nodes * 12
+    else:
+        bytes_per_launch = scene.n * 81 + nodes * 8
+    us = 1e3 * ms_dom / n_dom if n_dom else None
+    achieved = bytes_per_launch / (us * 1e-6) / 1e9 if n_dom else 0.0
+    ticks = run["kt_substeps"] / p.substeps
+    launches_per_step = n_dom / ticks
+    kernel_ms_per_step = us * 1e-3 * launches_per_step if n_dom else 0.0
+    traffic, source = (traffic_from_profile(p, workload) if dominant == "g2p2g" else (None, None))
+    return {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": source,
+            "bytes_per_launch": bytes_per_launch, "us_per_launch": us, "launches_per_step": launches_per_step,
+            "kernel_ms_per_step": kernel_ms_per_step, "kernel_share_of_step": kernel_ms_per_step / run["ms_per_step"],
+            "timing": f"HIP events around graph launches of each run of {dominant} launches over the timed window "
+                      f"of {run['kt_substeps']} substeps, replayed from the same initial state"}
+
+
+def kernels_of(run):
+    return {k: {"ms_total": round(ms, 4), "launches": n, "us_avg": round(1e3 * ms / n, 2) if n else None}
+            for k, (ms, n) in run["kt"].items()}
+
+
+def check_consistency(roof, ms_per_step, what):
+    """The dominant kernel's time per step cannot exceed the step it belongs to."""
+    if roof["kernel_ms_per_step"] > CONSISTENCY_SLACK * ms_per_step:
+        raise SystemExit(f"bench.py: {what}: {roof['kernel']} takes {roof['kernel_ms_per_step']:.3f} ms per step by "
+                         f"its events, more than the timed step of {ms_per_step:.3f} ms; the roofline would not "
+                         f"describe the timed ticks")
+
+
+def batched_object(args):
+    """The 64-scene batch (64 x 256^2 grids, 7,046,125 particles, 200 substeps per tick), timed like the
+    headline."""
+    preset, make_scene, description = WORKLOADS["batch64x256"]
+    p = abi.preset(preset)
+    scene = make_scene(1.0)
+    p.capacity = scene.n
+    steps = max(1, min(args.steps, 10))
+    run, x = timed_run(p, scene, steps, min(args.warmup, 2), label="batch64x256: ")
+    roof = roofline_of(p, scene, run, "batch64x256")
+    check_consistency(roof, run["ms_per_step"], "batch64x256")
+    return {"workload": description, "value": run["value"], "unit": "particle-substeps/s",
+            "ms_per_step": run["ms_per_step"], "steps": steps, "warmup": min(args.warmup, 2), "particles": scene.n,
+            "scenes": p.num_scenes, "substeps_per_step": p.substeps, "tile": [p.block_x, p.block_y], "chunk": p.chunk,
+            "sort_interval": p.sort_interval, "layout": scene_layout("batch64x256", scene, p),
+            "state_finite": bool(np.isfinite(x).all()),
+            "roofline": {k: roof[k] for k in ("kernel", "achieved", "frac", "us_per_launch", "launches_per_step",
+                                              "kernel_share_of_step")},
+            "kernels": kernels_of(run)}
+
+
+def learned_object(args):
+    """The learned closure on the 4,194,304-particle jelly + snow scene against the analytic engine on the
+    same scene: throughput of both over the same ticks, and the relative position and velocity errors of the
+    learned engine against the analytic one after one tick (100 substeps) from the initial state."""
+    preset, make_scene, description = WORKLOADS["learned1024"]
+    p = abi.preset(preset)
+    scene = make_scene(1.0)
+    p.capacity = scene.n
+    w = np.load(CLOSURE_WEIGHTS)
+    steps = max(1, min(args.steps, 10))
+    warm = min(args.warmup, 2)
+    run, _ = timed_run(p, scene, steps, warm, weights=w, label="learned1024: ")
+    roof = roofline_of(p, scene, run, "learned1024")
+    check_consistency(roof, run["ms_per_step"], "learned1024")
+    pa = p.copy()
+    pa.closure = abi.CLOSURE_ANALYTIC
+    plain, _ = timed_run(pa, scene, steps, warm, kernel_timing=False, label="learned1024 scene, analytic: ")
+    ref = {}
+    for name, q, weights in (("learned", p, w), ("plain", pa, None)):
+        with Engine(q) as e:
+            e.set_scene(scene)
+            if weights is not None:
+                e.set_closure(weights)
+            e.step(1)
+            e.sync()
+            ref[name] = e.readback(("x", "v"))
+
+    def rel(a, b):
+        a = np.asarray(a, np.float64)
+        b = np.asarray(b, np.float64)
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    ex, ev = rel(ref["learned"]["x"], ref["plain"]["x"]), rel(ref["learned"]["v"], ref["plain"]["v"])
+    ratio = run["value"] / plain["value"]
+    log(f"learned / plain {ratio:.3f}; after {p.substeps} substeps relative error vs plain: x {ex:.2e}, v {ev:.2e}")
+    return {"workload": description, "value": run["value"], "unit": "particle-substeps/s",
+            "ms_per_step": run["ms_per_step"], "steps": steps, "warmup": warm, "particles": scene.n,
+            "plain_value": plain["value"], "plain_ms_per_step": plain["ms_per_step"], "learned_over_plain": ratio,
+            "rel_position_error_vs_plain": ex, "rel_velocity_error_vs_plain": ev, "substeps_compared": p.substeps,
+            "layout": scene_layout("learned1024", scene, p),
+            "roofline": {k: roof[k] for k in ("kernel", "achieved", "frac", "us_per_launch", "launches_per_step",
+                                              "kernel_share_of_step")},
+            "mfma_tflops": scene.n * 2 * 64 * 64 / (roof["us_per_launch"] * 1e-6) / 1e12,
+            "kernels": kernels_of(run)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -140,6 +302,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--mode", choices=["reference", "tiled", "fused"], default="fused")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the batched and learned objects")
     ap.add_argument("--scale", type=float, default=1.0, help="particle count scale (1.0 = 4,194,304)")
     ap.add_argument("--autotune", action="store_true", help="pick tile shape, chunk and sort interval by mpm_autotune")
     ap.add_argument("--sort-interval", type=int, default=0, metavar="SUBSTEPS",
@@ -151,12 +314,12 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run device copies of MIB MiB (profiler counter calibration)")
     args = ap.parse_args()
-    if args.copy_calibration:
-        copy_calibration(args.copy_calibration)
     if args.gpus != 1:
         print(f"bench.py: the engine runs on one GPU; --gpus {args.gpus} is not supported (sharding across GPUs "
               f"is designed, not implemented)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
 
     preset, make_scene, description = WORKLOADS[args.workload]
     p = abi.preset(preset)
@@ -175,54 +338,20 @@ def main():
         log(f"autotune: {res.candidates} candidates in {time.perf_counter() - t0:.1f} s -> tile {res.block_x}x"
             f"{res.block_y}, chunk {res.chunk}, sort interval {res.sort_interval}, "
             f"{res.ms_per_substep * 1e3:.1f} us/substep")
-    eng = Engine(p)
-    eng.set_scene(scene)
-    if p.closure == abi.CLOSURE_LEARNED:
-        eng.set_closure(np.load(CLOSURE_WEIGHTS))
-        log(f"learned closure loaded from {os.path.relpath(CLOSURE_WEIGHTS, ROOT)}")
-    log("engine created, state resident in HBM")
-    for _ in range(args.warmup):
-        eng.step(1)
-    eng.sync()
-    log(f"warmup {args.warmup} ticks done")
-    # Per-kernel HIP-event timing on the engine's stream: two binning cycles of eagerly launched ticks (a
-    # cycle is one tick, or every tick up to the next binning when sort_interval spans several) right
-    # before and two right after the timed region (events cannot sit inside the tick graph).  The scene
-    # compresses under gravity and the fused kernel slows as it does, so they bracket the timed region and
-    # their sum stands for its average (single ticks vary by a few per cent).
-    cycle = 2 * p.substeps * (max(1, -(-p.sort_interval // p.substeps)) if p.mode != abi.MODE_REFERENCE else 1)
-    kt_before = eng.time_kernels(cycle)
-    t0 = time.perf_counter()
-    eng.step(args.steps)
-    eng.sync()
-    wall = time.perf_counter() - t0
-    last_tick_ms = eng.last_step_ms()
-    ms_per_step = wall * 1e3 / args.steps
-    value = scene.n * p.substeps * args.steps / wall
-    log(f"timed {args.steps} ticks: {ms_per_step:.3f} ms/tick (last tick by HIP events {last_tick_ms:.3f} ms)")
-    kt_after = eng.time_kernels(cycle)
-    kt = {k: (kt_before[k][0] + kt_after[k][0], kt_before[k][1] + kt_after[k][1]) for k in kt_before}
-    kernels = {k: {"ms_total": round(ms, 4), "launches": n, "us_avg": round(1e3 * ms / n, 2) if n else None}
-               for k, (ms, n) in kt.items()}
-    out = eng.readback()
-    eng.close()
-    finite = bool(np.all(np.isfinite(out["x"])))
+    weights = np.load(CLOSURE_WEIGHTS) if p.closure == abi.CLOSURE_LEARNED else None
+    run, x = timed_run(p, scene, args.steps, args.warmup, weights=weights)
+    roof = roofline_of(p, scene, run, args.workload)
+    kernels = kernels_of(run)
     log(f"kernels: {json.dumps(kernels)}")
+    log(f"{roof['kernel']}: {roof['us_per_launch']:.2f} us per launch x {roof['launches_per_step']:.1f} per step = "
+        f"{roof['kernel_ms_per_step']:.3f} ms of the {run['ms_per_step']:.3f} ms step; frac {roof['frac']:.3f}")
+    check_consistency(roof, run["ms_per_step"], args.workload)
 
     nodes = p.grid_w * p.grid_h * p.num_scenes
-    dominant = "g2p2g" if p.mode == abi.MODE_FUSED else ("p2g" if p.mode == abi.MODE_REFERENCE else "g2p")
-    ms_dom, n_dom = kt[dominant]
-    if dominant == "g2p2g":
-        bytes_per_launch = scene.n * G2P2G_BYTES_PER_PARTICLE + nodes * G2P2G_BYTES_PER_NODE
-    elif dominant == "p2g":
-        bytes_per_launch = scene.n * 53 + nodes * 12
-    else:
-        bytes_per_launch = scene.n * 81 + nodes * 8
-    achieved = bytes_per_launch / (ms_dom / n_dom * 1e-3) / 1e9 if n_dom else 0.0
-    substep_s = wall / (args.steps * p.substeps)
+    substep_s = run["ms_per_step"] * 1e-3 / p.substeps
     substep_bytes = scene.n * SUBSTEP_BYTES_PER_PARTICLE + nodes * SUBSTEP_BYTES_PER_NODE
     # the binning on its own line, amortised over the substeps that the event-timed cycles covered
-    ms_bin, n_bin = kt["binning"]
+    ms_bin, n_bin = run["kt"]["binning"]
     binning = None
     if n_bin:
         us_sort = 1e3 * ms_bin / n_bin
@@ -230,76 +359,42 @@ def main():
         binning = {"us_per_sort": us_sort, "bytes_per_sort": bin_bytes,
                    "achieved": bin_bytes / (us_sort * 1e-6) / 1e9, "unit": "GB/s",
                    "frac": bin_bytes / (us_sort * 1e-6) / 1e9 / HBM_PEAK_GBS,
-                   "substeps_per_sort": 2 * cycle / n_bin,
-                   "us_per_substep_amortised": 1e3 * ms_bin / (2 * cycle)}
+                   "substeps_per_sort": run["kt_substeps"] / n_bin,
+                   "us_per_substep_amortised": 1e3 * ms_bin / run["kt_substeps"]}
 
+    config = {"workload": description if args.scale == 1.0 else f"{args.workload} scaled x{args.scale}: {scene.n} particles",
+              "particles": scene.n, "scenes": p.num_scenes, "grid_cells": p.grid_w - 4,
+              "substeps_per_step": p.substeps, "mode": args.mode, "tile": [p.block_x, p.block_y], "chunk": p.chunk,
+              "sort_interval": p.sort_interval}
+    config.update(scene_layout(args.workload, scene, p))
     result = {
         "metric": METRIC,
-        "value": value,
+        "value": run["value"],
         "unit": "particle-substeps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": run["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": description if args.scale == 1.0 else
-                               f"{args.workload} scaled x{args.scale}: {scene.n} particles",
-                   "particles": scene.n, "scenes": p.num_scenes, "grid_cells": p.grid_w - 4,
-                   "substeps_per_step": p.substeps,
-                   "mode": args.mode, "tile": [p.block_x, p.block_y], "chunk": p.chunk,
-                   "sort_interval": p.sort_interval},
-        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic_from_profile(p, args.workload) if dominant == "g2p2g" else None,
-                     "bytes_per_launch": bytes_per_launch,
-                     "us_per_launch": 1e3 * ms_dom / n_dom if n_dom else None},
+        "config": config,
+        "roofline": roof,
         "substep": {"us": substep_s * 1e6, "algorithmic_bytes_unfused": substep_bytes,
-                    "effective_GBs": substep_bytes / substep_s / 1e9},
+                    "effective_GBs_unfused_model": substep_bytes / substep_s / 1e9},
         "binning": binning,
         "kernels": kernels,
-        "state_finite": finite,
+        "state_finite": bool(np.isfinite(x).all()),
     }
-    if p.closure == abi.CLOSURE_LEARNED:
-        result["vs_plain"] = compare_plain(p, scene, args, value)
-        result["roofline"]["mfma_tflops"] = scene.n * 2 * 64 * 64 / (ms_dom / n_dom * 1e-3) / 1e12 if n_dom else None
+    if args.workload == "mixed1024" and args.scale == 1.0 and not args.no_extra:
+        result["batched"] = batched_object(args)
+        result["learned"] = learned_object(args)
     if not args.no_cpu_baseline:
         log("cpu baseline (FP64 oracle) ...")
         result["cpu_baseline"] = cpu_baseline(p, scene)
     print(json.dumps(result), flush=True)
-
-
-def compare_plain(p, scene, args, learned_value):
-    """The same scene with the analytic closure: throughput over the same number of ticks, and the relative
-    position error of the learned closure after one tick from the initial state."""
-    pa = p.copy()
-    pa.closure = abi.CLOSURE_ANALYTIC
-    with Engine(pa) as e:
-        e.set_scene(scene)
-        e.step(1)
-        e.sync()
-        x_plain = e.readback()["x"].astype(np.float64)
-        for _ in range(args.warmup):
-            e.step(1)
-        e.sync()
-        t0 = time.perf_counter()
-        e.step(args.steps)
-        e.sync()
-        plain_value = scene.n * pa.substeps * args.steps / (time.perf_counter() - t0)
-    with Engine(p) as e:
-        e.set_scene(scene)
-        e.set_closure(np.load(CLOSURE_WEIGHTS))
-        e.step(1)
-        e.sync()
-        x_learned = e.readback()["x"].astype(np.float64)
-    err = float(np.linalg.norm(x_learned - x_plain) / np.linalg.norm(x_plain))
-    log(f"plain (analytic) on the same scene: {plain_value:.4g} particle-substeps/s; learned / plain "
-        f"{learned_value / plain_value:.3f}; relative position error after {p.substeps} substeps {err:.2e}")
-    return {"plain_value": plain_value, "learned_over_plain": learned_value / plain_value,
-            "rel_position_error_vs_plain": err, "substeps_compared": p.substeps}
 
 
 if __name__ == "__main__":

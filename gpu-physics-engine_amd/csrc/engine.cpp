@@ -270,17 +270,20 @@ int32_t allocate(MpmEngine* h)
     return MPM_OK;
 }
 
-// Optional per-launch HIP event pairs (mpm_time_kernels); absent during graph capture.
+// Per-class timing of mpm_time_kernels; absent while the tick graphs are captured.  Consecutive launches of
+// one class form a group (the fused substeps of a segment are one group).  Each group is captured from the
+// stream into a graph of its own and launched between two HIP events, so its kernels run with the dispatch
+// gaps of a graph launch, as in the tick graph, and no event packet sits between the kernels of a group.
+// The events skip the system-scope release a default event performs (an L2 write-back that would
+// otherwise be counted inside the bracketed kernels).
 struct LaunchTimer {
     hipStream_t st = nullptr;
     std::vector<hipEvent_t> ev;  // one (start, end) pair per group
     std::vector<int> cls;        // class of each group
     std::vector<int> count;      // launches in each group
-    bool open = false;           // the last group still has no end event
-    // Consecutive launches of one class form a group bracketed by one event pair, so back-to-back runs
-    // (the fused substeps of a segment) carry no event packets between their kernels.  The events skip the
-    // system-scope release a default event performs (an L2 write-back that would otherwise be counted
-    // inside the bracketed kernels).
+    std::vector<hipGraph_t> graphs;
+    std::vector<hipGraphExec_t> execs;
+    bool open = false;           // the last group is still being captured
     hipError_t begin(int k)
     {
         if (open && cls.back() == k) {
@@ -301,18 +304,37 @@ struct LaunchTimer {
         ev.push_back(b);
         cls.push_back(k);
         count.push_back(1);
-        open = true;
-        return hipEventRecord(a, st);
+        e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+        open = e == hipSuccess;
+        return e;
     }
     hipError_t end() { return hipSuccess; }  // the group ends when another class starts (or at close)
+    // Ends the capture of the open group and launches it between its events.
     hipError_t close()
     {
         if (!open) { return hipSuccess; }
         open = false;
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(st, &g);
+        if (e != hipSuccess) { return e; }
+        graphs.push_back(g);
+        hipGraphExec_t x = nullptr;
+        if ((e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0)) != hipSuccess) { return e; }
+        execs.push_back(x);
+        if ((e = hipGraphUpload(x, st)) != hipSuccess) { return e; }
+        if ((e = hipEventRecord(ev[ev.size() - 2], st)) != hipSuccess) { return e; }
+        if ((e = hipGraphLaunch(x, st)) != hipSuccess) { return e; }
         return hipEventRecord(ev.back(), st);
     }
     ~LaunchTimer()
     {
+        if (open) {  // a failed enqueue left a capture open: end it so the stream is usable again
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(st, &g);
+            if (g) { (void)hipGraphDestroy(g); }
+        }
+        for (hipGraphExec_t x : execs) { (void)hipGraphExecDestroy(x); }
+        for (hipGraph_t g : graphs) { (void)hipGraphDestroy(g); }
         for (hipEvent_t e : ev) { (void)hipEventDestroy(e); }
     }
 };

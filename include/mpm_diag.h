@@ -33,9 +33,10 @@ extern "C" {
  * *n receives it.  Not available in MPM_MODE_REFERENCE (returns MPM_ERR_STATE). */
 int32_t mpm_debug_bin(MpmEngine* h, uint32_t* keys, int32_t* ids, int32_t* n);
 
-/* Advance `substeps` substeps with the handle's mode, launching eagerly (not through the tick graph) and
- * bracketing every run of consecutive launches of one class (for example the fused substeps of a
- * segment, which run back to back) with a pair of HIP events on the engine stream.  ms[k] receives the
+/* Advance `substeps` substeps with the handle's mode, outside the tick graph: every run of consecutive
+ * launches of one class (for example the fused substeps of a segment, which run back to back) is captured
+ * into a graph of its own and launched between a pair of HIP events on the engine stream, so the kernels
+ * run with a graph launch's dispatch gaps, as inside the tick graph.  ms[k] receives the
  * summed GPU time of class k, launches[k] the number of launches (arrays of MPM_KT_COUNT).  The output
  * slot is not written.  Advances the simulation exactly like mpm_step would: whole ticks follow mpm_step's
  * binning schedule (when sort_interval spans several ticks, only some ticks start with a binning); a
