@@ -1227,11 +1227,12 @@ int32_t mpm_time_kernels(MpmEngine* h, int32_t substeps, float* ms, int32_t* lau
     return MPM_OK;
 }
 
-int32_t mpm_compute_stress(MpmEngine* h, int32_t n, const uint8_t* material, const float* F, const float* J,
-                           float* tau)
+namespace {
+// Stress diagnostics on host arrays (row-major F[4n], J[n], tau[3n]); snow_return: snow particles take the
+// return mapping first and F_out / J_out (host, may be NULL without snow_return) receive the result.
+int32_t stress_impl(MpmEngine* h, int32_t n, const uint8_t* material, const float* F, const float* J, float* tau,
+                    int snow_return, float* F_out, float* J_out)
 {
-    if (!h || n < 0 || (n > 0 && (!material || !F || !J || !tau))) { return MPM_ERR_INVALID_ARG; }
-    if (n == 0) { return MPM_OK; }
     const size_t N = (size_t)n;
     std::vector<float> planes(5 * N);
     for (size_t i = 0; i < N; ++i) {
@@ -1239,24 +1240,48 @@ int32_t mpm_compute_stress(MpmEngine* h, int32_t n, const uint8_t* material, con
         planes[4 * N + i] = J[i];
     }
     uint8_t* buf = nullptr;
-    const size_t bytes = 8 * N * sizeof(float) + N;
+    const size_t bytes = 13 * N * sizeof(float) + N;
     HIP_TRY(h, hipMalloc((void**)&buf, bytes));
     float* dF = reinterpret_cast<float*>(buf);
     float* dJ = dF + 4 * N;
     float* dT = dJ + N;
-    uint8_t* dM = reinterpret_cast<uint8_t*>(dT + 3 * N);
+    float* dFo = dT + 3 * N;
+    float* dJo = dFo + 4 * N;
+    uint8_t* dM = reinterpret_cast<uint8_t*>(dJo + N);
     hipError_t e = hipMemcpy(dF, planes.data(), 5 * N * sizeof(float), hipMemcpyHostToDevice);
     if (e == hipSuccess) { e = hipMemcpy(dM, material, N, hipMemcpyHostToDevice); }
-    if (e == hipSuccess) { e = launch_stress(h->stream, h->dp, n, dM, dF, dJ, dT); }
+    if (e == hipSuccess) { e = launch_stress(h->stream, h->dp, n, dM, dF, dJ, dT, snow_return, dFo, dJo); }
     if (e == hipSuccess) { e = hipStreamSynchronize(h->stream); }
-    std::vector<float> t(3 * N);
-    if (e == hipSuccess) { e = hipMemcpy(t.data(), dT, 3 * N * sizeof(float), hipMemcpyDeviceToHost); }
+    std::vector<float> t(8 * N);
+    if (e == hipSuccess) { e = hipMemcpy(t.data(), dT, 8 * N * sizeof(float), hipMemcpyDeviceToHost); }
     (void)hipFree(buf);
     if (e != hipSuccess) { return fail(h, MPM_ERR_DEVICE, (int32_t)e, "stress kernel"); }
     for (size_t i = 0; i < N; ++i) {
         for (int k = 0; k < 3; ++k) { tau[3 * i + k] = t[k * N + i]; }
+        if (F_out) {
+            for (int k = 0; k < 4; ++k) { F_out[4 * i + k] = t[(3 + k) * N + i]; }
+        }
+        if (J_out) { J_out[i] = t[7 * N + i]; }
     }
     return MPM_OK;
+}
+}  // namespace
+
+int32_t mpm_compute_stress(MpmEngine* h, int32_t n, const uint8_t* material, const float* F, const float* J,
+                           float* tau)
+{
+    if (!h || n < 0 || (n > 0 && (!material || !F || !J || !tau))) { return MPM_ERR_INVALID_ARG; }
+    if (n == 0) { return MPM_OK; }
+    return stress_impl(h, n, material, F, J, tau, 0, nullptr, nullptr);
+}
+
+int32_t mpm_compute_snow_return(MpmEngine* h, int32_t n, const float* F_trial, const float* Jp, float* F_out,
+                                float* Jp_out, float* tau)
+{
+    if (!h || n < 0 || (n > 0 && (!F_trial || !Jp || !F_out || !Jp_out || !tau))) { return MPM_ERR_INVALID_ARG; }
+    if (n == 0) { return MPM_OK; }
+    std::vector<uint8_t> snow((size_t)n, (uint8_t)MPM_MAT_SNOW);
+    return stress_impl(h, n, snow.data(), F_trial, Jp, tau, 1, F_out, Jp_out);
 }
 
 }  // extern "C"

@@ -86,9 +86,10 @@ hipError_t launch_emitter(hipStream_t st, const DevParams& p, const Store& s, in
                           float oy, float velx, float vely, int width, float spacing, float jitter, int n_emit,
                           uint64_t seed, float inv_tick);
 
-// Per-particle Kirchhoff stress (t00, t01, t11) of F (4 planes) and J for material mat[i].
+// Per-particle Kirchhoff stress (t00, t01, t11) of F (4 planes) and J for material mat[i]; with snow_return,
+// snow particles take the G2P return mapping first (projected F and Jp into F_out, J_out, 4 planes and 1).
 hipError_t launch_stress(hipStream_t st, const DevParams& p, int n, const uint8_t* mat, const float* F, const float* J,
-                         float* tau);
+                         float* tau, int snow_return, float* F_out, float* J_out);
 
 // Deinterleave / interleave helpers for the boundary copies.
 hipError_t launch_scatter_to_user(hipStream_t st, const Store& s, const int32_t* count, int cap, float* x2, float* v2,

@@ -1511,40 +1511,47 @@ hipError_t launch_emitter(hipStream_t st, const DevParams& p, const Store& s, in
 
 // ===================================================================================== stress
 
+// Stress diagnostic: kirchhoff_tau of each particle, the function every P2G kernel calls.  With
+// snow_return set, snow particles first take the return mapping of G2P (snow_project on F as the trial
+// elastic F and J as Jp, writing the projected F and Jp to F_out, J_out) and their stress then uses the
+// mapping's rotation, as k_g2p2g does.
 __global__ __launch_bounds__(kThreads) void k_stress(DevParams p, int n, const uint8_t* mat, const float* F,
-                                                     const float* J, float* tau)
+                                                     const float* J, float* tau, int snow_return, float* F_out,
+                                                     float* J_out)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) { return; }
-    const Mat2 f = {F[i], F[n + i], F[2 * n + i], F[3 * n + i]};
+    Mat2 f = {F[i], F[n + i], F[2 * n + i], F[3 * n + i]};
+    float j = J[i];
     const int m = mat[i];
-    float t00, t01, t11;
-    if (m == MPM_MAT_FLUID) {
-        const float p_hat = p.tait_k * (powf(1.f / J[i], p.tait_gamma) - 1.f);
-        const float pr = fminf(fmaxf(p_hat, p.p_min), p.p_max);
-        t00 = -J[i] * pr;
-        t01 = 0.f;
-        t11 = -J[i] * pr;
-    } else {
-        float mu = p.mu, lambda = p.lambda;
-        if (m == MPM_MAT_SNOW) {
-            const float h = __expf(p.hardening * (1.f - J[i]));
-            mu *= h;
-            lambda *= h;
-        }
-        corotated_tau(f, mu, lambda, t00, t01, t11);
+    float2 rot;
+    const float2* rp = nullptr;
+    if (snow_return && m == MPM_MAT_SNOW) {
+        float rc, rs;
+        snow_project(p, f, j, rc, rs);
+        rot = make_float2(rc, rs);
+        rp = &rot;
     }
+    float t00, t01, t11;
+    kirchhoff_tau(p, m, f, j, rp, t00, t01, t11);
     tau[i] = t00;
     tau[n + i] = t01;
     tau[2 * n + i] = t11;
+    if (F_out) {
+        F_out[i] = f.a00;
+        F_out[n + i] = f.a01;
+        F_out[2 * n + i] = f.a10;
+        F_out[3 * n + i] = f.a11;
+        J_out[i] = j;
+    }
 }
 
 hipError_t launch_stress(hipStream_t st, const DevParams& p, int n, const uint8_t* mat, const float* F, const float* J,
-                         float* tau)
+                         float* tau, int snow_return, float* F_out, float* J_out)
 {
     if (n <= 0) { return hipSuccess; }
     hipLaunchKernelGGL(k_stress, dim3((unsigned)((n + kThreads - 1) / kThreads)), dim3(kThreads), 0, st, p, n, mat, F,
-                       J, tau);
+                       J, tau, snow_return, F_out, J_out);
     return hipGetLastError();
 }
 

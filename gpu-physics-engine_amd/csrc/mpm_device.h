@@ -248,25 +248,33 @@ __device__ __forceinline__ void snow_project(const DevParams& p, Mat2& F, float&
     F.a11 = u10 * n0 * v10 + u11 * n1 * v11;
 }
 
-// Affine momentum matrix Q of P2G.  Fluid follows the reference expression order
-// (MPM.cu:68-80: Tait pressure on J, V = V0 J, coeff = -dt V 4 inv_dx^2, Q = m C + coeff sigma);
-// jelly and snow use Q = m C - dt V0 4 inv_dx^2 tau with tau the fixed-corotated Kirchhoff stress.
+// Tait pressure of the tracked J (MPM.cu:68-69): p = k ((1/J)^gamma - 1) clamped to [p_min, p_max].  1/J is
+// the hardware reciprocal (1 ulp; every G2P clamps J to [j_min, j_max], j_min > 0); gamma = 1 (linear bulk
+// modulus) skips powf, whose result would be 1/J exactly.
+__device__ __forceinline__ float tait_pressure(const DevParams& p, float J)
+{
+    const float inv_j = __builtin_amdgcn_rcpf(J);
+    const float p_hat = p.tait_k * ((p.tait_gamma == 1.f ? inv_j : powf(inv_j, p.tait_gamma)) - 1.f);
+    return fminf(fmaxf(p_hat, p.p_min), p.p_max);
+}
+
+// Kirchhoff stress tau = (t00, t01 = t10, t11) of one particle: the stress of every P2G kernel and of the
+// stress diagnostic, so both run the same instructions.
+//   fluid: tau = J sigma = -J p I with the Tait pressure p (the reference's stress, MPM.cu:68-80, where
+//          Q = m C + (-dt V0 J 4 inv_dx^2)(-p I) = m C - dt V0 4 inv_dx^2 tau);
+//   jelly: fixed-corotated on F with the polar rotation of F;
+//   snow:  the same with mu, lambda scaled by exp(hardening (1 - Jp)) (Stomakhin et al. 2013).
 // snow_rot: for a snow particle, the polar rotation of F when the caller holds it from the return mapping
 // (g2p_update), else nullptr and it is computed here.
-__device__ __forceinline__ Mat2 affine_q(const DevParams& p, int mat, const Mat2& C, const Mat2& F, float J,
-                                         const float2* snow_rot = nullptr)
+__device__ __forceinline__ void kirchhoff_tau(const DevParams& p, int mat, const Mat2& F, float J,
+                                              const float2* snow_rot, float& t00, float& t01, float& t11)
 {
     if (mat == MPM_MAT_FLUID) {
-        // (1/J)^gamma; gamma = 1 (linear bulk modulus) skips powf, whose result would be 1/J exactly.  1/J is
-        // the hardware reciprocal (1 ulp; every G2P clamps J to [j_min, j_max], j_min > 0).
-        const float inv_j = __builtin_amdgcn_rcpf(J);
-        const float p_hat = p.tait_k * ((p.tait_gamma == 1.f ? inv_j : powf(inv_j, p.tait_gamma)) - 1.f);
-        const float pressure = fminf(fmaxf(p_hat, p.p_min), p.p_max);
-        const float V = p.vol * J;
-        const float coeff = -p.dt * V * 4.f * p.inv_dx * p.inv_dx;
-        const float sig = -pressure;
-        return {p.mass * C.a00 + coeff * sig, p.mass * C.a01 + coeff * 0.f, p.mass * C.a10 + coeff * 0.f,
-                p.mass * C.a11 + coeff * sig};
+        const float t = -J * tait_pressure(p, J);
+        t00 = t;
+        t01 = 0.f;
+        t11 = t;
+        return;
     }
     float mu = p.mu, lambda = p.lambda;
     if (mat == MPM_MAT_SNOW) {
@@ -281,8 +289,16 @@ __device__ __forceinline__ Mat2 affine_q(const DevParams& p, int mat, const Mat2
     } else {
         polar_cs(F, c, s);
     }
-    float t00, t01, t11;
     corotated_tau_r(F, c, s, mu, lambda, t00, t01, t11);
+}
+
+// Affine momentum matrix Q = m C - dt V0 4 inv_dx^2 tau of P2G (MPM.cu:71-80) with the stress of
+// kirchhoff_tau.
+__device__ __forceinline__ Mat2 affine_q(const DevParams& p, int mat, const Mat2& C, const Mat2& F, float J,
+                                         const float2* snow_rot = nullptr)
+{
+    float t00, t01, t11;
+    kirchhoff_tau(p, mat, F, J, snow_rot, t00, t01, t11);
     const float D = p.dt * p.vol * 4.f * p.inv_dx * p.inv_dx;
     return {p.mass * C.a00 - D * t00, p.mass * C.a01 - D * t01, p.mass * C.a10 - D * t01, p.mass * C.a11 - D * t11};
 }

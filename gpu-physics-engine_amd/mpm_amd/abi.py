@@ -176,6 +176,7 @@ SIGNATURES = {
     "mpm_debug_bin": (ctypes.c_int32, [_vp, _P(ctypes.c_uint32), _i32p, _i32p]),
     "mpm_time_kernels": (ctypes.c_int32, [_vp, ctypes.c_int32, _f32p, _i32p]),
     "mpm_compute_stress": (ctypes.c_int32, [_vp, ctypes.c_int32, _u8p, _f32p, _f32p, _f32p]),
+    "mpm_compute_snow_return": (ctypes.c_int32, [_vp, ctypes.c_int32, _f32p, _f32p, _f32p, _f32p, _f32p]),
 }
 
 CLOSURE_NUM_WEIGHTS = 4675

@@ -214,6 +214,19 @@ class Engine:
         _check(self.lib, self.h, rc, "mpm_compute_stress")
         return tau
 
+    def snow_return(self, F_trial, Jp):
+        """G2P return mapping of snow particles then their P2G stress with the mapping's rotation, by the
+        engine's device code: (F float[n,4], Jp float[n], tau float[n,3])."""
+        F = np.ascontiguousarray(F_trial, dtype=np.float32).reshape(-1, 4)
+        n = F.shape[0]
+        J = np.ascontiguousarray(Jp, dtype=np.float32).reshape(n)
+        Fo = np.zeros((n, 4), np.float32)
+        Jo = np.zeros(n, np.float32)
+        tau = np.zeros((n, 3), np.float32)
+        rc = self.lib.mpm_compute_snow_return(self.h, n, _fp(F), _fp(J), _fp(Fo), _fp(Jo), _fp(tau))
+        _check(self.lib, self.h, rc, "mpm_compute_snow_return")
+        return Fo, Jo, tau
+
 
 def tune_key(params, scene, substeps=None):
     """What an autotuned launch shape depends on: grid, scenes, particle count (to the nearest power of two),

@@ -43,11 +43,19 @@ int32_t mpm_debug_bin(MpmEngine* h, uint32_t* keys, int32_t* ids, int32_t* n);
  * remainder shorter than a tick starts with one. */
 int32_t mpm_time_kernels(MpmEngine* h, int32_t substeps, float* ms, int32_t* launches);
 
-/* Kirchhoff stress of n particles with the handle's material parameters, computed by the engine's
- * device code: F float[4n] row-major, J float[n] (Jp for snow), material uint8[n]; tau float[3n] =
- * (tau00, tau01, tau11) per particle. */
+/* Kirchhoff stress of n particles with the handle's material parameters, computed by the device function
+ * every P2G kernel calls (fluid: -J p I with the Tait pressure of MPM.cu:68-69; jelly and snow:
+ * fixed-corotated with the polar rotation of F): F float[4n] row-major, J float[n] (Jp for snow), material
+ * uint8[n]; tau float[3n] = (tau00, tau01, tau11) per particle. */
 int32_t mpm_compute_stress(MpmEngine* h, int32_t n, const uint8_t* material, const float* F, const float* J,
                            float* tau);
+
+/* The snow step of a fused substep on n particles: the return mapping of G2P on the trial elastic F
+ * (F_trial float[4n], Jp float[n]) -> projected F_out float[4n] and Jp_out float[n], then the P2G stress
+ * tau float[3n] of the projected state with the rotation the return mapping found, as the fused kernel
+ * computes it. */
+int32_t mpm_compute_snow_return(MpmEngine* h, int32_t n, const float* F_trial, const float* Jp, float* F_out,
+                                float* Jp_out, float* tau);
 
 #ifdef __cplusplus
 }

@@ -396,8 +396,8 @@ def test_batch64x256_full_size(oracle_mod):
 
 
 def test_stress_kernel_against_oracle(oracle_mod):
-    """Device fixed-corotated / snow / Tait stress against the FP64 oracle, including near-degenerate F
-    (det F < 0.1, the FP64 polar path) and inverted F."""
+    """Device fixed-corotated / snow / Tait stress (kirchhoff_tau, the function every P2G kernel calls)
+    against the FP64 oracle, including near-degenerate F (det F < 0.1, the FP64 polar path) and inverted F."""
     from test_oracle import rand_F
     p = abi.preset(abi.PRESET_MIXED1024)
     rng = np.random.default_rng(21)
@@ -414,12 +414,17 @@ def test_stress_kernel_against_oracle(oracle_mod):
     assert np.max(np.abs(tau - ref3) / scale) < 1e-4
 
 
-def test_tait_pressure_within_ulp_bound(oracle_mod):
-    """Tait pressure with gamma = 3.2 (powf, MPM.cu:68-69) against FP64 std::pow, per particle, within the
-    FP32 error budget of its steps: 1/J rounded (relative 2^-24, amplified gamma times by the power), powf
-    (at most 2 ulp), k(.. - 1) and -J p rounded (2^-24 each, taken as 2^-22 of |tau|).  J covers both clamps
-    (J > 1 gives p_hat < 0 -> p_min = -0; J = 0.1 gives p_hat ~ 1.3e8 -> p_max) and J = 1 exactly."""
-    p = abi.preset(abi.PRESET_REFERENCE)
+@pytest.mark.parametrize("preset", [abi.PRESET_REFERENCE, abi.PRESET_MIXED1024], ids=["gamma3.2", "gamma1"])
+def test_tait_pressure_within_ulp_bound(oracle_mod, preset):
+    """Fluid stress tau = -J p with the Tait pressure (MPM.cu:68-69) of the shipped device function
+    (tait_pressure in kirchhoff_tau: the hardware reciprocal for 1/J, then powf for gamma != 1, the power
+    skipped for gamma = 1) against FP64, per particle, within the FP32 error budget of its steps: 1/J from
+    v_rcp_f32 (at most 1 ulp, relative 2^-23, amplified gamma times by the power), powf (at most 2 ulp),
+    k(.. - 1) and -J p rounded (2^-24 each, taken as 2^-22 of |tau|): (2 gamma + 4) 2^-24 of k (1/J)^gamma J
+    plus 2^-22 |tau|.  Both presets: the reference's gamma = 3.2 through powf and the unit-square presets'
+    gamma = 1.  J covers both clamps (J > 1 gives p_hat < 0 -> p_min = -0; J = 0.1 gives p_hat above p_max for
+    gamma = 3.2) and J = 1 exactly."""
+    p = abi.preset(preset)
     rng = np.random.default_rng(33)
     J = np.concatenate([rng.uniform(0.55, 1.45, 3000), [1.0, 0.1, 0.25, 0.5, 2.0, 10.0]]).astype(np.float32)
     n = J.size
@@ -433,14 +438,53 @@ def test_tait_pressure_within_ulp_bound(oracle_mod):
     pw = (1.0 / J64) ** np.float64(p.tait_gamma)
     gamma, k = float(np.float32(p.tait_gamma)), float(np.float32(p.tait_k))
     unclamped = (k * (pw - 1.0) > float(p.p_min)) & (k * (pw - 1.0) < float(p.p_max))
-    bound = np.where(unclamped, J64 * k * pw * (gamma + 4.0) * 2.0**-24, 0.0) + 2.0**-22 * np.abs(ref[:, 0])
+    bound = np.where(unclamped, J64 * k * pw * (2.0 * gamma + 4.0) * 2.0**-24, 0.0) + 2.0**-22 * np.abs(ref[:, 0])
     err = np.abs(tau[:, 0].astype(np.float64) - ref[:, 0])
     assert np.all(err <= bound), (J[np.argmax(err - bound)], np.max(err - bound))
     assert np.array_equal(tau[:, 0], tau[:, 2])
-    # the clamps: J > 1 has zero pressure, J = 0.1 sits at p_max = 8.2e5 (tau = -J p_max)
+    # the clamps: J > 1 has zero pressure; J = 1: (1/J)^gamma - 1 = 0 exactly
     assert np.all(tau[J > 1.0, 0] == 0.0)
-    assert tau[n - 6 + 1, 0] == np.float32(-0.1) * np.float32(p.p_max)
-    assert tau[n - 6, 0] == 0.0  # J = 1: (1/J)^gamma - 1 = 0 exactly
+    assert tau[n - 6, 0] == 0.0
+    if preset == abi.PRESET_REFERENCE:  # J = 0.1 sits at p_max = 8.2e5 (tau = -J p_max)
+        assert tau[n - 6 + 1, 0] == np.float32(-0.1) * np.float32(p.p_max)
+
+
+def test_snow_return_mapping_stress_against_oracle(oracle_mod):
+    """The snow step of the fused kernel (mpm_compute_snow_return): the G2P return mapping on trial elastic F
+    (singular values clamped to [1 - theta_c, 1 + theta_s], the removed volume moved into Jp; Stomakhin et al.
+    2013), then the P2G stress of the projected F computed with the rotation the mapping found (as k_g2p2g
+    does), against the FP64 oracle's snow_project followed by its stress.  Trial singular values in [0.9, 1.1]
+    cross both clamp bounds; 200 particles have det F < 0.1 (FP64 polar path) and 200 are inverted."""
+    from test_oracle import rand_F
+    p = abi.preset(abi.PRESET_MIXED1024)
+    rng = np.random.default_rng(23)
+    n = 4000
+    F = rand_F(rng, n, lo=0.9, hi=1.1)
+    F[:200] = rand_F(rng, 200, lo=0.05, hi=0.3)
+    F[200:400] = rand_F(rng, 200, lo=0.9, hi=1.1, allow_inverted=True)
+    Jp = rng.uniform(0.8, 1.2, n)
+    F32, Jp32 = F.astype(np.float32), Jp.astype(np.float32)
+    with Engine(p) as e:
+        Fo, Jo, tau = e.snow_return(F32, Jp32)
+    Fr, Jr = oracle_mod.snow_project(p, F32.astype(np.float64), Jp32.astype(np.float64))
+    ref = oracle_mod.tau(p, np.full(n, abi.MAT_SNOW, np.uint8), Fr, Jr)[:, [0, 1, 3]]
+    lo, hi = 1.0 - float(np.float32(p.theta_c)), 1.0 + float(np.float32(p.theta_s))
+    sv = np.linalg.svd(F.reshape(n, 2, 2), compute_uv=False)
+    assert (sv < lo).any() and (sv > hi).any() and ((sv > lo) & (sv < hi)).any()
+    err_F = np.abs(Fo - Fr).max(axis=1) / np.abs(Fr).max(axis=1)
+    err_J = np.abs(Jo - Jr) / np.abs(Jr)
+    # Stress error in units of the hardened moduli h (2 mu + lambda), h = exp(xi (1 - Jp)): tau = 2 mu h (F - R)
+    # F^T + lambda h (J - 1) J, so rounding F and R at FP32 (about 5e-7 measured in F) moves tau by about that
+    # many units however small the strain; relative to |tau| itself the error grows as the strain vanishes.
+    h = np.exp(float(np.float32(p.hardening)) * (1.0 - Jr))
+    unit = h * (2.0 * float(np.float32(p.mu)) + float(np.float32(p.lam)))
+    err_t = np.abs(tau - ref).max(axis=1) / unit
+    rel_t = np.abs(tau - ref).max(axis=1) / (np.abs(ref).max(axis=1) + p.mu * 1e-3)
+    print(f"[snow return] max F {err_F.max():.2e} Jp {err_J.max():.2e} tau/moduli {err_t.max():.2e} "
+          f"tau/|tau| {rel_t.max():.2e}", flush=True)
+    assert err_F.max() < 2e-6
+    assert err_J.max() < 2e-6
+    assert err_t.max() < 5e-6
 
 
 def test_output_view_and_emitter():

@@ -145,6 +145,42 @@ def test_grid_update_walls_and_gravity(oracle_mod):
     assert np.array_equal(out[..., 2], g[..., 2])
 
 
+def test_grid_update_wall_rule_hand_built_cases(oracle_mod):
+    """The wall rule of gridKernel (MPM.cu:159-168) on hand-built nodes, with the answers written out rather
+    than computed: with wall = 2 on a 20-node grid, nodes 0..2 and 17..19 are wall nodes along each axis,
+    node 3 and node 16 are not.  At a wall node only the outward component is zeroed (v_x < 0 at the left,
+    v_x > 0 at the right; the same for y); an inward component passes unchanged; a non-wall node keeps
+    both signs.  Gravity is off so every expected value is exact: mom = (+-0.5, +-0.25), mass 2."""
+    p = unit_params(cells=16)
+    p.gravity_x = p.gravity_y = 0.0
+    W, H = p.grid_w, p.grid_h
+    assert (W, H, p.wall) == (20, 20, 2)
+    g = np.zeros((H, W, 3))
+    # (x node, y node, sign of mom_x, sign of mom_y) -> expected (v_x, v_y)
+    cases = {
+        (2, 10, -1, +1): (0.0, 0.125),    # left wall, outward x zeroed
+        (2, 11, +1, -1): (0.25, -0.125),  # left wall, inward x kept
+        (3, 10, -1, +1): (-0.25, 0.125),  # first interior node: kept
+        (17, 10, +1, -1): (0.0, -0.125),  # right wall (x = W - 3), outward x zeroed
+        (17, 11, -1, +1): (-0.25, 0.125),  # right wall, inward kept
+        (16, 10, +1, -1): (0.25, -0.125),  # x = W - 4: interior, kept
+        (10, 2, +1, -1): (0.25, 0.0),     # bottom wall, outward y zeroed
+        (11, 2, -1, +1): (-0.25, 0.125),  # bottom wall, inward kept
+        (10, 3, +1, -1): (0.25, -0.125),  # y = 3: interior
+        (10, 17, -1, +1): (-0.25, 0.0),   # top wall (y = H - 3), outward y zeroed
+        (11, 17, +1, -1): (0.25, -0.125),  # top wall, inward kept
+        (10, 16, -1, +1): (-0.25, 0.125),  # y = H - 4: interior
+        (0, 0, -1, -1): (0.0, 0.0),       # corner: both outward, both zeroed
+        (19, 19, -1, -1): (-0.25, -0.125),  # opposite corner: both inward, both kept
+    }
+    for (x, y, sx, sy) in cases:
+        g[y, x] = [0.5 * sx, 0.25 * sy, 2.0]
+    out = oracle_mod.grid_update(p, g)
+    for (x, y, sx, sy), (vx, vy) in cases.items():
+        assert (out[y, x, 0], out[y, x, 1]) == (vx, vy), (x, y)
+        assert out[y, x, 2] == 2.0
+
+
 # ---------------------------------------------------------------------------------------------- G2P
 
 def test_g2p_reproduces_linear_field(oracle_mod):
