@@ -1215,13 +1215,13 @@ k_g2p2g(
         Mat2 C, F = ld_F(s, o);
         float J = ldo(s.J, o);
         gather<BX, BY>(p, g, gb, ox, oy, sv, px, py, vx, vy, C);
-        float2 snow_rot = make_float2(1.f, 0.f);  // a snow particle's polar rotation, from its return mapping
-        g2p_update(p, mat, px, py, vx, vy, C, F, J, &snow_rot);
+        SnowHint snow = {1.f, 0.f, 1.f};  // a snow particle's rotation and hardening, from its return mapping
+        g2p_update(p, mat, px, py, vx, vy, C, F, J, &snow);
         sto(s.x, o, px);
         sto(s.y, o, py);
         st_F(s, o, F);
         sto(s.J, o, J);
-        const Mat2 Q = affine_q(p, mat, C, F, J, &snow_rot);
+        const Mat2 Q = affine_q(p, mat, C, F, J, &snow);
         scatter<BX, BY>(p, g, gb, ox, oy, wtile, px, py, vx, vy, Q);
     }
     __syncthreads();
@@ -1524,13 +1524,13 @@ __global__ __launch_bounds__(kThreads) void k_stress(DevParams p, int n, const u
     Mat2 f = {F[i], F[n + i], F[2 * n + i], F[3 * n + i]};
     float j = J[i];
     const int m = mat[i];
-    float2 rot;
-    const float2* rp = nullptr;
+    SnowHint hint;
+    const SnowHint* rp = nullptr;
     if (snow_return && m == MPM_MAT_SNOW) {
         float rc, rs;
         snow_project(p, f, j, rc, rs);
-        rot = make_float2(rc, rs);
-        rp = &rot;
+        hint = {rc, rs, snow_hardening(p, j)};
+        rp = &hint;
     }
     float t00, t01, t11;
     kirchhoff_tau(p, m, f, j, rp, t00, t01, t11);

@@ -102,17 +102,23 @@ struct Mat2 {
 // ---- B-spline weights (MPM.cu:92-110) ----
 // The reference's gridCoord x * inv_dx + halo (MPM.cu:84) is contracted to one fused multiply-add by
 // nvcc; it is written as an explicit fmaf here so the base node (and the bin key) is reproducible.
+// With h = g - 0.5 (exact for the grid coordinates here), base = floor(h) and f = g - base = fract(h) + 0.5, so
+// the weights' arguments 1.5 - f = 1 - r, f - 1 = r - 0.5 and f - 0.5 = r come from r = fract(h) exactly
+// (r is a multiple of ulp(h), so 1 - r is exact).  base is taken by truncation, which equals floor for
+// h >= 0; for h < 0 (positions left of the domain, possible only for emitted particles before their first
+// out-of-bounds handling) truncation and floor both give a base <= 0, and every caller clamps the base to
+// [0, dim - 3], so the clamped base is the same.  f and the weights use r either way.
 __device__ __forceinline__ void bspline(float x, float inv_dx, float halo, int& base, float& f, float w[3])
 {
-    const float g = fmaf(x, inv_dx, halo);
-    base = (int)floorf(g - 0.5f);
-    f = g - (float)base;
-    const float a = 1.5f - f;
-    const float b = f - 1.0f;
-    const float c = f - 0.5f;
+    const float h = fmaf(x, inv_dx, halo) - 0.5f;
+    base = (int)h;
+    const float r = __builtin_amdgcn_fractf(h);
+    f = r + 0.5f;
+    const float a = 1.f - r;
+    const float b = r - 0.5f;
     w[0] = 0.5f * a * a;
     w[1] = 0.75f - b * b;
-    w[2] = 0.5f * c * c;
+    w[2] = 0.5f * r * r;
 }
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
@@ -258,16 +264,23 @@ __device__ __forceinline__ float tait_pressure(const DevParams& p, float J)
     return fminf(fmaxf(p_hat, p.p_min), p.p_max);
 }
 
+// What the return mapping of a snow particle leaves for its stress: the polar rotation (c, s) of the projected F
+// and the hardening factor h = exp(hardening (1 - Jp)) of its new Jp.
+struct SnowHint {
+    float c, s, h;
+};
+__device__ __forceinline__ float snow_hardening(const DevParams& p, float Jp) { return __expf(p.hardening * (1.f - Jp)); }
+
 // Kirchhoff stress tau = (t00, t01 = t10, t11) of one particle: the stress of every P2G kernel and of the
 // stress diagnostic, so both run the same instructions.
 //   fluid: tau = J sigma = -J p I with the Tait pressure p (the reference's stress, MPM.cu:68-80, where
 //          Q = m C + (-dt V0 J 4 inv_dx^2)(-p I) = m C - dt V0 4 inv_dx^2 tau);
 //   jelly: fixed-corotated on F with the polar rotation of F;
 //   snow:  the same with mu, lambda scaled by exp(hardening (1 - Jp)) (Stomakhin et al. 2013).
-// snow_rot: for a snow particle, the polar rotation of F when the caller holds it from the return mapping
-// (g2p_update), else nullptr and it is computed here.
+// snow: for a snow particle, the rotation and hardening factor when the caller holds them from the return
+// mapping (g2p_update), else nullptr and they are computed here (the same expressions).
 __device__ __forceinline__ void kirchhoff_tau(const DevParams& p, int mat, const Mat2& F, float J,
-                                              const float2* snow_rot, float& t00, float& t01, float& t11)
+                                              const SnowHint* snow, float& t00, float& t01, float& t11)
 {
     if (mat == MPM_MAT_FLUID) {
         const float t = -J * tait_pressure(p, J);
@@ -277,16 +290,18 @@ __device__ __forceinline__ void kirchhoff_tau(const DevParams& p, int mat, const
         return;
     }
     float mu = p.mu, lambda = p.lambda;
-    if (mat == MPM_MAT_SNOW) {
-        const float h = __expf(p.hardening * (1.f - J));
-        mu *= h;
-        lambda *= h;
-    }
     float c, s;
-    if (mat == MPM_MAT_SNOW && snow_rot) {
-        c = snow_rot->x;
-        s = snow_rot->y;
+    if (mat == MPM_MAT_SNOW && snow) {
+        c = snow->c;
+        s = snow->s;
+        mu *= snow->h;
+        lambda *= snow->h;
     } else {
+        if (mat == MPM_MAT_SNOW) {
+            const float h = snow_hardening(p, J);
+            mu *= h;
+            lambda *= h;
+        }
         polar_cs(F, c, s);
     }
     corotated_tau_r(F, c, s, mu, lambda, t00, t01, t11);
@@ -295,10 +310,10 @@ __device__ __forceinline__ void kirchhoff_tau(const DevParams& p, int mat, const
 // Affine momentum matrix Q = m C - dt V0 4 inv_dx^2 tau of P2G (MPM.cu:71-80) with the stress of
 // kirchhoff_tau.
 __device__ __forceinline__ Mat2 affine_q(const DevParams& p, int mat, const Mat2& C, const Mat2& F, float J,
-                                         const float2* snow_rot = nullptr)
+                                         const SnowHint* snow = nullptr)
 {
     float t00, t01, t11;
-    kirchhoff_tau(p, mat, F, J, snow_rot, t00, t01, t11);
+    kirchhoff_tau(p, mat, F, J, snow, t00, t01, t11);
     const float D = p.dt * p.vol * 4.f * p.inv_dx * p.inv_dx;
     return {p.mass * C.a00 - D * t00, p.mass * C.a01 - D * t01, p.mass * C.a10 - D * t01, p.mass * C.a11 - D * t11};
 }
@@ -327,10 +342,10 @@ __device__ __forceinline__ void handle_oob(const DevParams& p, float& px, float&
 
 // Particle update after the gather (MPM.cu:252-265): advect, out-of-bounds, F <- (I + dt C) F, then
 // J <- clamp(J det A) or, for snow, the return mapping on F and Jp.  C = 4 inv_dx^2 B (MPM.cu:255) comes from
-// the caller's gather (formed before the OOB handling, as in the reference).  For snow, *snow_rot (if given)
-// receives the polar rotation of the new F, for affine_q.
+// the caller's gather (formed before the OOB handling, as in the reference).  For snow, *snow (if given)
+// receives the polar rotation of the new F and the hardening factor of the new Jp, for affine_q.
 __device__ __forceinline__ void g2p_update(const DevParams& p, int mat, float& px, float& py, float& vx, float& vy,
-                                           const Mat2& C, Mat2& F, float& J, float2* snow_rot = nullptr)
+                                           const Mat2& C, Mat2& F, float& J, SnowHint* snow = nullptr)
 {
     px += vx * p.dt;
     py += vy * p.dt;
@@ -340,7 +355,7 @@ __device__ __forceinline__ void g2p_update(const DevParams& p, int mat, float& p
     if (mat == MPM_MAT_SNOW) {
         float rc, rs;
         snow_project(p, F, J, rc, rs);
-        if (snow_rot) { *snow_rot = make_float2(rc, rs); }
+        if (snow) { *snow = {rc, rs, snow_hardening(p, J)}; }
     } else {
         J *= det2(A);
         J = fminf(fmaxf(J, p.j_min), p.j_max);

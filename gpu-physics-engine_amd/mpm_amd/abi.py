@@ -8,7 +8,8 @@ import ctypes
 import os
 
 _PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(_PKG_DIR, "lib", "libmpm_engine.so")
+# MPM_ENGINE_LIB selects another build of the same sources (for example to compare two builds in one session).
+LIB_PATH = os.environ.get("MPM_ENGINE_LIB") or os.path.join(_PKG_DIR, "lib", "libmpm_engine.so")
 
 MPM_OK = 0
 MPM_BUSY = 1
