@@ -52,21 +52,19 @@ __device__ __forceinline__ float fast_atan2(float y, float x)
 }
 
 // Closure features of one particle; returns false when outside the trained range (then x is unspecified).
+// The SVD comes in closed form: with E = (F00 + F11) / 2, D = (F00 - F11) / 2, G = (F10 + F01) / 2 and
+// H = (F10 - F01) / 2, F = U diag(s0, s1) V^T with s0 = q + r >= s1 = q - r for q = |(E, H)|, r = |(D, G)|,
+// and U the rotation by (atan2(H, E) + atan2(G, D)) / 2.  tau = U diag U^T is unchanged by U -> -U, so the
+// angle is reduced to (-pi/2, pi/2]; at s0 = s1 any U serves (there tau is isotropic).  In the trained range
+// q >= 0.6 (q is at least the mean singular value), so the hardware square root needs no denormal scaling.
 __device__ __forceinline__ bool closure_features(const Mat2& F, float jh, float x[4])
 {
-    float uc, us, s0, s1, vc, vs, rc, rs;
-    svd2(F, uc, us, s0, s1, vc, vs, rc, rs);
-    (void)rc;  // the features use U and the singular values only
-    (void)rs;
-    // canonical order s0 >= s1: swapping the singular values rotates U by 90 degrees (tau is unchanged)
-    float th = fast_atan2(us, uc);
-    if (s1 > s0) {
-        const float t = s0;
-        s0 = s1;
-        s1 = t;
-        th += 1.57079632679f;
-    }
-    // tau = U diag U^T is invariant under U -> -U: reduce the angle to (-pi/2, pi/2]
+    const float E = 0.5f * (F.a00 + F.a11), D = 0.5f * (F.a00 - F.a11);
+    const float G = 0.5f * (F.a10 + F.a01), H = 0.5f * (F.a10 - F.a01);
+    const float q = __builtin_amdgcn_sqrtf(fmaf(E, E, H * H));
+    const float r = __builtin_amdgcn_sqrtf(fmaf(D, D, G * G));
+    const float s0 = q + r, s1 = q - r;
+    float th = 0.5f * (fast_atan2(H, E) + fast_atan2(G, D));
     if (th > 1.57079632679f) { th -= 3.14159265359f; }
     if (th <= -1.57079632679f) { th += 3.14159265359f; }
     x[0] = (s0 - 1.1f) * 2.f;
