@@ -6,9 +6,10 @@ velocity uniform in [-1,1]^2, dt = 2e-5, particle density 16.  One step = one ti
 workload's run length), launched as one hipGraph; particle state is resident in HBM before the timed region.
 
 Prints one JSON line: throughput, per-step time, the roofline of the dominant kernel timed with HIP events on
-the engine's stream, the FP64 oracle timed on this host's cores (bounded sample), and two more workloads
-timed the same way: the 64-scene batch (`batched`) and the learned closure on the 4,194,304-particle jelly +
-snow scene against the analytic engine on the same scene (`learned`).
+the engine's stream, the FP64 oracle timed on this host's cores (bounded sample), and three more workloads
+timed the same way: the 64-scene batch (`batched`), the learned closure on the 4,194,304-particle jelly +
+snow scene against the analytic engine on the same scene (`learned`), and the learned closure on 16 scenes of
+2048^2 cells (91.5M particles) against the analytic engine on the same scenes (`learned_batched`).
 """
 import argparse
 import json
@@ -250,22 +251,23 @@ def batched_object(args):
             "kernels": kernels_of(run)}
 
 
-def learned_object(args):
-    """The learned closure on the 4,194,304-particle jelly + snow scene against the analytic engine on the
-    same scene: throughput of both over the same ticks, and the relative position and velocity errors of the
-    learned engine against the analytic one after one tick (100 substeps) from the initial state."""
-    preset, make_scene, description = WORKLOADS["learned1024"]
+def learned_object(args, workload="learned1024", max_steps=10):
+    """A learned-closure workload against the analytic engine on the same scene: throughput of both over the
+    same ticks, and the relative position and velocity errors of the learned engine against the analytic one
+    after one tick (100 substeps) from the initial state.  learned1024 is the 4,194,304-particle jelly + snow
+    scene; learned16x2048 is the 16-scene batch of 2048^2 grids (91.5M particles)."""
+    preset, make_scene, description = WORKLOADS[workload]
     p = abi.preset(preset)
     scene = make_scene(1.0)
     p.capacity = scene.n
     w = np.load(CLOSURE_WEIGHTS)
-    steps = max(1, min(args.steps, 10))
+    steps = max(1, min(args.steps, max_steps))
     warm = min(args.warmup, 2)
-    run, _ = timed_run(p, scene, steps, warm, weights=w, label="learned1024: ")
-    roof = roofline_of(p, scene, run, "learned1024")
+    run, _ = timed_run(p, scene, steps, warm, weights=w, label=f"{workload}: ")
+    roof = roofline_of(p, scene, run, workload)
     pa = p.copy()
     pa.closure = abi.CLOSURE_ANALYTIC
-    plain, _ = timed_run(pa, scene, steps, warm, kernel_timing=False, label="learned1024 scene, analytic: ")
+    plain, _ = timed_run(pa, scene, steps, warm, kernel_timing=False, label=f"{workload} scene, analytic: ")
     ref = {}
     for name, q, weights in (("learned", p, w), ("plain", pa, None)):
         with Engine(q) as e:
@@ -281,17 +283,22 @@ def learned_object(args):
         b = np.asarray(b, np.float64)
         return float(np.linalg.norm(a - b) / np.linalg.norm(b))
     ex, ev = rel(ref["learned"]["x"], ref["plain"]["x"]), rel(ref["learned"]["v"], ref["plain"]["v"])
+    del ref
     ratio = run["value"] / plain["value"]
-    log(f"learned / plain {ratio:.3f}; after {p.substeps} substeps relative error vs plain: x {ex:.2e}, v {ev:.2e}")
-    return {"workload": description, "value": run["value"], "unit": "particle-substeps/s",
-            "ms_per_step": run["ms_per_step"], "steps": steps, "warmup": warm, "particles": scene.n,
-            "plain_value": plain["value"], "plain_ms_per_step": plain["ms_per_step"], "learned_over_plain": ratio,
-            "rel_position_error_vs_plain": ex, "rel_velocity_error_vs_plain": ev, "substeps_compared": p.substeps,
-            "layout": scene_layout("learned1024", scene, p),
-            "roofline": {k: roof[k] for k in ("kernel", "achieved", "frac", "us_per_launch", "launches_per_step",
-                                              "kernel_share_of_step")},
-            "mfma_tflops": scene.n * 2 * 64 * 64 / (roof["us_per_launch"] * 1e-6) / 1e12,
-            "kernels": kernels_of(run)}
+    log(f"{workload}: learned / plain {ratio:.3f}; after {p.substeps} substeps relative error vs plain: "
+        f"x {ex:.2e}, v {ev:.2e}")
+    out = {"workload": description, "value": run["value"], "unit": "particle-substeps/s",
+           "ms_per_step": run["ms_per_step"], "steps": steps, "warmup": warm, "particles": scene.n,
+           "scenes": p.num_scenes, "plain_value": plain["value"], "plain_ms_per_step": plain["ms_per_step"],
+           "learned_over_plain": ratio, "rel_position_error_vs_plain": ex, "rel_velocity_error_vs_plain": ev,
+           "substeps_compared": p.substeps, "layout": scene_layout(workload, scene, p),
+           "roofline": {k: roof[k] for k in ("kernel", "achieved", "frac", "us_per_launch", "launches_per_step",
+                                             "kernel_share_of_step")},
+           "mfma_tflops": scene.n * 2 * 64 * 64 / (roof["us_per_launch"] * 1e-6) / 1e12,
+           "kernels": kernels_of(run)}
+    if p.num_scenes > 1:
+        out.update({"tile": [p.block_x, p.block_y], "chunk": p.chunk, "sort_interval": p.sort_interval})
+    return out
 
 
 def main():
@@ -301,7 +308,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--mode", choices=["reference", "tiled", "fused"], default="fused")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-extra", action="store_true", help="skip the batched and learned objects")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the batched, learned and learned_batched objects")
     ap.add_argument("--scale", type=float, default=1.0, help="particle count scale (1.0 = 4,194,304)")
     ap.add_argument("--autotune", action="store_true", help="pick tile shape, chunk and sort interval by mpm_autotune")
     ap.add_argument("--sort-interval", type=int, default=0, metavar="SUBSTEPS",
@@ -390,6 +398,7 @@ def main():
     if args.workload == "mixed1024" and args.scale == 1.0 and not args.no_extra:
         result["batched"] = batched_object(args)
         result["learned"] = learned_object(args)
+        result["learned_batched"] = learned_object(args, "learned16x2048", max_steps=3)
     if not args.no_cpu_baseline:
         log("cpu baseline (FP64 oracle) ...")
         result["cpu_baseline"] = cpu_baseline(p, scene)
