@@ -141,13 +141,14 @@ __device__ __forceinline__ void chain_operands(const floatx16& acc, bf16x8& lo, 
 
 // Network output (tau / mu) of the particle of every lane.  Must be called by all 64 lanes
 // (wave-uniform control flow); x holds the lane's own features.  Particles are the N dimension of the
-// MFMAs: N-tile nt holds particles 32 nt + l%32, evaluated one N-tile at a time.
+// MFMAs: N-tile nt holds particles 32 nt + l%32, evaluated one N-tile after the other.  The N-tile loop is
+// unrolled so that nt is a constant: a runtime nt costs selects on every operand and result it picks.
 __device__ __forceinline__ void closure_eval_wave(const ClosurePacked* s, const float x[4], float y[3])
 {
     const int lane = threadIdx.x & 63;
     const int half = lane >> 5;
     float res[2][3];
-#pragma unroll 1
+#pragma unroll
     for (int nt = 0; nt < 2; ++nt) {
         // layer 1 B operand: (x_hi, x_lo) of particle 32 nt + l%32 in both halves, so that
         // A1 B1 = W1hi (x_hi + x_lo) + W1lo (x_hi + x_lo)
