@@ -246,8 +246,8 @@ def batched_object(args):
             "scenes": p.num_scenes, "substeps_per_step": p.substeps, "tile": [p.block_x, p.block_y], "chunk": p.chunk,
             "sort_interval": p.sort_interval, "layout": scene_layout("batch64x256", scene, p),
             "state_finite": bool(np.isfinite(x).all()),
-            "roofline": {k: roof[k] for k in ("kernel", "achieved", "frac", "us_per_launch", "launches_per_step",
-                                              "kernel_share_of_step")},
+            "roofline": {k: roof[k] for k in ("kernel", "achieved", "frac", "traffic", "traffic_source",
+                                              "us_per_launch", "launches_per_step", "kernel_share_of_step")},
             "kernels": kernels_of(run)}
 
 
@@ -292,8 +292,8 @@ def learned_object(args, workload="learned1024", max_steps=10):
            "scenes": p.num_scenes, "plain_value": plain["value"], "plain_ms_per_step": plain["ms_per_step"],
            "learned_over_plain": ratio, "rel_position_error_vs_plain": ex, "rel_velocity_error_vs_plain": ev,
            "substeps_compared": p.substeps, "layout": scene_layout(workload, scene, p),
-           "roofline": {k: roof[k] for k in ("kernel", "achieved", "frac", "us_per_launch", "launches_per_step",
-                                             "kernel_share_of_step")},
+           "roofline": {k: roof[k] for k in ("kernel", "achieved", "frac", "traffic", "traffic_source",
+                                             "us_per_launch", "launches_per_step", "kernel_share_of_step")},
            "mfma_tflops": scene.n * 2 * 64 * 64 / (roof["us_per_launch"] * 1e-6) / 1e12,
            "kernels": kernels_of(run)}
     if p.num_scenes > 1:
