@@ -360,7 +360,7 @@ def test_modes_agree_full_size():
 
 
 def test_mixed1024_long_run_stays_finite_and_conserves_mass():
-    """The 4,194,304-particle mixed scene for 60 ticks (6000 substeps, binning every second tick, the boxes
+    """The 4,194,304-particle mixed scene for 60 ticks (6000 substeps, binning every tick, the boxes
     falling and piling on the floor): every particle stays finite and inside the domain, none is lost, and
     the grid mass of the last substep equals N m."""
     p = abi.preset(abi.PRESET_MIXED1024)
