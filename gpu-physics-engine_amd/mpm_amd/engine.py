@@ -91,7 +91,10 @@ class Engine:
                 np.ascontiguousarray(a, dtype=dt).reshape(n)
         v, C, F, J = arr(v, 2), arr(C, 4), arr(F, 4), arr(J, 1)
         m = arr(material, 1, np.uint8)
-        sc = None if scene_counts is None else np.ascontiguousarray(scene_counts, dtype=np.int32)
+        sc = None if scene_counts is None else np.ascontiguousarray(scene_counts, dtype=np.int32).reshape(-1)
+        if sc is not None and sc.size != self.params.num_scenes:
+            # mpm_set_particles reads num_scenes counts
+            raise ValueError(f"scene_counts has {sc.size} entries, the handle has {self.params.num_scenes} scenes")
         rc = self.lib.mpm_set_particles(
             self.h, n, _fp(x), _fp(v), _fp(C), _fp(F), _fp(J),
             None if m is None else m.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
