@@ -617,6 +617,9 @@ def test_ragged_batch_with_empty_and_single_particle_scenes(oracle_mod):
     v = rng.uniform(-1, 1, (n, 2)).astype(np.float32)
     mat = rng.integers(0, 3, n).astype(np.uint8)
     p.capacity = n
+    with Engine(p) as e:  # one count per scene, or the upload is refused before the C-ABI reads them
+        with pytest.raises(ValueError):
+            e.set_particles(x, v, material=mat, scene_counts=np.array(counts[:3], np.int32))
     out = run_gpu(p, x, v, mat, scene_counts=np.array(counts, np.int32))
     lo = 0
     for k, c in enumerate(counts):
