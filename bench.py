@@ -45,13 +45,15 @@ CONSISTENCY_SLACK = 1.02
 
 
 def binning_bytes_per_particle(p):
-    """Algorithmic bytes per particle of one binning: keys from x, y (read 8, write 4); P radix passes of 8
-    bits over the bin bits (the histogram reads the key 4, the scatter reads and writes (key, id) 16); the
-    bin ranges read the key 4; the layered order reads the key twice and the id and writes both (20); the
-    full-state gather reads the id 4 and 57 B of state and writes 57 B."""
+    """Algorithmic bytes per particle of one steady-state binning: keys from x, y (read 8, write 4); P radix
+    passes of 8 bits over the bin bits (the histogram reads the key 4, the scatter reads and writes (key, id)
+    16); the bin ranges read the key 4; the layered order reads the key twice and the id and writes both
+    (20); the gather reads the permutation 4 and moves the state: 57 B each way, or 33 B in the fused mode,
+    whose ticks bin after their first P2G and leave v and C (24 B) behind."""
     bins = p.num_scenes * (-(-p.grid_w // p.block_x)) * (-(-p.grid_h // p.block_y))
     passes = -(-max(1, (bins - 1).bit_length()) // 8)
-    return 12 + 20 * passes + 4 + 20 + 4 + 2 * 57
+    state = 33 if p.mode == abi.MODE_FUSED else 57
+    return 12 + 20 * passes + 4 + 20 + 4 + 2 * state
 
 
 WORKLOADS = {

@@ -24,17 +24,26 @@
 
 using namespace mpm;
 
+namespace {
+// How a tick starts its binning.  kBinAfterP2G: the tick's first P2G runs on the bins of the tick before
+// (valid: the population and its order are unchanged since), then the binning gathers every component but v
+// and C, which no kernel reads before the tick's last G2P writes them.  kBinFirst: a full binning before
+// the first P2G (after an upload or emission, whose particles are in no bin, and in the non-fused modes).
+enum BinStart { kBinNone = 0, kBinFirst = 1, kBinAfterP2G = 2 };
+}  // namespace
+
 struct MpmEngine {
     MpmParams params{};
     DevParams dp{};
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t t0 = nullptr, t1 = nullptr, done = nullptr;
-    // Tick graphs by starting store c and by whether the tick begins with a binning ([c][1]) or reuses the
-    // bins of an earlier tick ([c][0], built only when sort_interval spans several ticks).
-    hipGraph_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
-    hipGraphExec_t exec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
-    int flips[2] = {0, 0};      // store swaps per tick (parity) of each variant
+    // Tick graphs by starting store c and by how the tick bins (BinStart): [c][kBinNone] reuses the bins of an
+    // earlier tick (built only when sort_interval spans several ticks), [c][kBinFirst] bins before its first
+    // P2G, [c][kBinAfterP2G] (fused mode) runs its first P2G on the bins of the tick before and bins after it.
+    hipGraph_t graph[2][3] = {};
+    hipGraphExec_t exec[2][3] = {};
+    int flips[3] = {0, 0, 0};   // store swaps per tick (parity) of each variant
     int cur = 0;                // store holding the current state
     int ticks_per_sort = 1;     // ceil(sort_interval / substeps) in the binned modes
     int ticks_since_sort = 0;   // ticks run since the last tick that began with a binning
@@ -361,7 +370,7 @@ Grid grid_for(const MpmEngine* h, int target, int src, int zero)
 // result), then the output copy when `output` is set.  dirty: which accumulator sets hold data, before (in,
 // all zero) and after (out); every segment checks it scatters only into zero sets.
 hipError_t enqueue_substeps(MpmEngine* h, int& c, int substeps, bool output, LaunchTimer* tm, bool dirty[3],
-                            bool bin_first = true)
+                            BinStart bin = kBinFirst)
 {
     const MpmParams& p = h->params;
     const DevParams& d = h->dp;
@@ -387,12 +396,13 @@ hipError_t enqueue_substeps(MpmEngine* h, int& c, int substeps, bool output, Lau
         int done = 0;
         while (done < substeps) {
             const int L = std::min(p.sort_interval, substeps - done);
-            if (bin_first || done > 0) {
+            const bool after_p2g = done == 0 && bin == kBinAfterP2G && p.mode == MPM_MODE_FUSED;
+            if ((bin != kBinNone || done > 0) && !after_p2g) {
                 ENQ(MPM_KT_BINNING, launch_binning(st, d, h->store[c], h->store[c ^ 1], h->count, h->ss));
                 c ^= 1;
             }
-            const Store& s = h->store[c];
             if (p.mode == MPM_MODE_TILED) {
+                const Store& s = h->store[c];
                 for (int k = 0; k < L; ++k) {
                     ENQ(MPM_KT_P2G, launch_p2g_tiled(st, h->shape, d, s, h->grid, h->ss, h->count));
                     ENQ(MPM_KT_GRID, launch_grid_update(st, d, h->grid));
@@ -405,8 +415,14 @@ hipError_t enqueue_substeps(MpmEngine* h, int& c, int substeps, bool output, Lau
                 // after); the closing grid update reads the last set, zeroes it and the one before.
                 if (dirty[0] || dirty[1] || dirty[2]) { return hipErrorUnknown; }  // plan invariant
                 int t = 0, prev = -1;
-                ENQ(MPM_KT_P2G, launch_p2g_tiled(st, h->shape, d, s, grid_for(h, t, -1, -1), h->ss, h->count));
+                ENQ(MPM_KT_P2G, launch_p2g_tiled(st, h->shape, d, h->store[c], grid_for(h, t, -1, -1), h->ss,
+                                                 h->count));
                 dirty[t] = true;
+                if (after_p2g) {  // the grid sums do not depend on the particles' order
+                    ENQ(MPM_KT_BINNING, launch_binning(st, d, h->store[c], h->store[c ^ 1], h->count, h->ss, false));
+                    c ^= 1;
+                }
+                const Store& s = h->store[c];
                 for (int k = 1; k < L; ++k) {
                     const int src = t;
                     t = k % 3;
@@ -431,11 +447,15 @@ hipError_t enqueue_substeps(MpmEngine* h, int& c, int substeps, bool output, Lau
     return hipSuccess;
 }
 
-// Whether the next tick starts with a binning: always when sort_interval fits in a tick, otherwise every
-// ticks_per_sort ticks and whenever the population changed outside a tick (set, load, emission).
-bool next_tick_bins(const MpmEngine* h)
+// How the next tick bins: not at all while the bins of an earlier tick serve (sort_interval spanning several
+// ticks); after its first P2G when the bins of the tick before match the particles (fused mode); first
+// otherwise: after the population changed outside a tick (set, load, emission), before any binning, in the
+// non-fused modes.
+BinStart next_tick_bins(const MpmEngine* h)
 {
-    return h->ticks_per_sort <= 1 || h->need_sort || h->ticks_since_sort + 1 >= h->ticks_per_sort;
+    const bool bins = h->ticks_per_sort <= 1 || h->need_sort || h->ticks_since_sort + 1 >= h->ticks_per_sort;
+    if (!bins) { return kBinNone; }
+    return h->need_sort || h->params.mode != MPM_MODE_FUSED ? kBinFirst : kBinAfterP2G;
 }
 void note_tick(MpmEngine* h, bool binned)
 {
@@ -457,15 +477,17 @@ int32_t build_graphs(MpmEngine* h)
         segs = (p.substeps + iv - 1) / iv;
         h->ticks_per_sort = (p.sort_interval + p.substeps - 1) / p.substeps;
     }
-    h->flips[1] = segs & 1;
-    h->flips[0] = (segs - 1) & 1;
-    for (int b = 1; b >= 0; --b) {
-        if (b == 0 && h->ticks_per_sort <= 1) { break; }
+    h->flips[kBinFirst] = segs & 1;
+    h->flips[kBinAfterP2G] = segs & 1;
+    h->flips[kBinNone] = (segs - 1) & 1;
+    for (int b = kBinNone; b <= kBinAfterP2G; ++b) {
+        if (b == kBinNone && h->ticks_per_sort <= 1) { continue; }
+        if (b == kBinAfterP2G && p.mode != MPM_MODE_FUSED) { continue; }
         for (int c = 0; c < 2; ++c) {
             HIP_TRY(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
             int cc = c;
             bool dirty[3] = {false, false, false};
-            hipError_t e = enqueue_substeps(h, cc, p.substeps, true, nullptr, dirty, b == 1);
+            hipError_t e = enqueue_substeps(h, cc, p.substeps, true, nullptr, dirty, (BinStart)b);
             hipGraph_t g = nullptr;
             hipError_t e2 = hipStreamEndCapture(h->stream, &g);
             if (e != hipSuccess) {
@@ -487,7 +509,7 @@ void release(MpmEngine* h)
     if (!h) { return; }
     if (h->stream) { (void)hipStreamSynchronize(h->stream); }
     for (int c = 0; c < 2; ++c) {
-        for (int b = 0; b < 2; ++b) {
+        for (int b = 0; b < 3; ++b) {
             if (h->exec[c][b]) { (void)hipGraphExecDestroy(h->exec[c][b]); }
             if (h->graph[c][b]) { (void)hipGraphDestroy(h->graph[c][b]); }
         }
@@ -821,12 +843,12 @@ int32_t mpm_step(MpmEngine* h, int32_t n_ticks)
     HIP_TRY(h, set_output_view_async(h->stream, &h->staged[slot]));
     for (int32_t k = 0; k < n_ticks; ++k) {
         const bool last = k == n_ticks - 1;
-        const int bin = next_tick_bins(h) ? 1 : 0;
+        const BinStart bin = next_tick_bins(h);
         if (last) { HIP_TRY(h, hipEventRecord(h->t0, h->stream)); }
         HIP_TRY(h, hipGraphLaunch(h->exec[h->cur][bin], h->stream));
         if (last) { HIP_TRY(h, hipEventRecord(h->t1, h->stream)); }
         h->cur ^= h->flips[bin];
-        note_tick(h, bin);
+        note_tick(h, bin != kBinNone);
         h->has_ticked = true;
     }
     h->slot_count[slot].store(h->last_count, std::memory_order_relaxed);
@@ -1206,9 +1228,10 @@ int32_t mpm_time_kernels(MpmEngine* h, int32_t substeps, float* ms, int32_t* lau
     hipError_t e = hipSuccess;
     for (int left = substeps; e == hipSuccess && left > 0;) {
         const int n = std::min(left, h->params.substeps);
-        const bool bin = n < h->params.substeps || next_tick_bins(h);
+        BinStart bin = next_tick_bins(h);
+        if (n < h->params.substeps && bin == kBinNone) { bin = kBinFirst; }
         e = enqueue_substeps(h, c, n, false, &tm, dirty, bin);
-        note_tick(h, bin);
+        note_tick(h, bin != kBinNone);
         left -= n;
     }
     if (e == hipSuccess) { e = tm.close(); }

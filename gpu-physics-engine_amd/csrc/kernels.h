@@ -65,9 +65,10 @@ hipError_t launch_p2g_reference(hipStream_t st, const DevParams& p, const Store&
 hipError_t launch_g2p_reference(hipStream_t st, const DevParams& p, const Store& s, const int32_t* count, int cap,
                                 const Grid& g);
 
-// Stable binning: src (current order) -> dst (binned order), then bin ranges and work items.
+// Stable binning: src (current order) -> dst (binned order), then bin ranges and work items.  full = false
+// leaves v and C out of the gather (see k_gather).
 hipError_t launch_binning(hipStream_t st, const DevParams& p, const Store& src, const Store& dst,
-                          const int32_t* count, const SortScratch& ss);
+                          const int32_t* count, const SortScratch& ss, bool full = true);
 
 hipError_t launch_p2g_tiled(hipStream_t st, int shape, const DevParams& p, const Store& s, const Grid& g,
                             const SortScratch& ss, const int32_t* count);

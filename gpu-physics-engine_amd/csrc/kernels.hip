@@ -470,10 +470,13 @@ __global__ __launch_bounds__(kThreads) void k_radix_downsweep(const uint32_t* ke
     }
 }
 
-// Permutation gather of the whole particle state: dst[j] = src[perm[j]] (keys already in dst.key).
+// Permutation gather of the particle state: dst[j] = src[perm[j]] (keys already in dst.key).  With Full
+// false, v and C stay behind: a binning placed after a substep's P2G, whose next reader of v and C is the
+// G2P that rewrites them.
 // Work groups are dispatched round-robin over the 8 XCDs; the grid is a multiple of 8 and XCD x takes the
 // x-th contiguous eighth of the destination, so the source lines of a bin (whose particles stay close to
 // their previous positions) are fetched into one XCD's L2 rather than into all eight.
+template <bool Full>
 __global__ __launch_bounds__(kThreads) void k_gather(Store src, Store dst, const int32_t* perm, const int32_t* count)
 {
     const unsigned per = gridDim.x >> 3;
@@ -485,12 +488,14 @@ __global__ __launch_bounds__(kThreads) void k_gather(Store src, Store dst, const
     if ((unsigned)i >= (unsigned)n) { return; }
     dst.x[j] = src.x[i];
     dst.y[j] = src.y[i];
-    dst.vx[j] = src.vx[i];
-    dst.vy[j] = src.vy[i];
-    dst.c00[j] = src.c00[i];
-    dst.c01[j] = src.c01[i];
-    dst.c10[j] = src.c10[i];
-    dst.c11[j] = src.c11[i];
+    if (Full) {
+        dst.vx[j] = src.vx[i];
+        dst.vy[j] = src.vy[i];
+        dst.c00[j] = src.c00[i];
+        dst.c01[j] = src.c01[i];
+        dst.c10[j] = src.c10[i];
+        dst.c11[j] = src.c11[i];
+    }
     dst.f00[j] = src.f00[i];
     dst.f01[j] = src.f01[i];
     dst.f10[j] = src.f10[i];
@@ -696,7 +701,7 @@ __global__ __launch_bounds__(kThreads) void k_items_write(const int32_t* bstart,
 }
 
 hipError_t launch_binning(hipStream_t st, const DevParams& p, const Store& src, const Store& dst,
-                          const int32_t* count, const SortScratch& ss)
+                          const int32_t* count, const SortScratch& ss, bool full)
 {
     const int cap = ss.capacity;
     const unsigned pblocks = (unsigned)((cap + kThreads - 1) / kThreads);
@@ -731,7 +736,13 @@ hipError_t launch_binning(hipStream_t st, const DevParams& p, const Store& src, 
     hipLaunchKernelGGL(k_layer_bins, dim3((unsigned)ss.num_keys), dim3(kThreads), layer_bins_lds_bytes(p.node_bits), st,
                        kin, vin, ss.bin_start,
                        ss.bin_end, ss.num_keys, p.node_bits, p.bin_x, dst.key, ss.perm);
-    hipLaunchKernelGGL(k_gather, dim3((pblocks + 7u) / 8u * 8u), dim3(kThreads), 0, st, src, dst, ss.perm, count);
+    if (full) {
+        hipLaunchKernelGGL(k_gather<true>, dim3((pblocks + 7u) / 8u * 8u), dim3(kThreads), 0, st, src, dst, ss.perm,
+                           count);
+    } else {
+        hipLaunchKernelGGL(k_gather<false>, dim3((pblocks + 7u) / 8u * 8u), dim3(kThreads), 0, st, src, dst, ss.perm,
+                           count);
+    }
     const unsigned kblocks = (unsigned)((ss.num_keys + kThreads - 1) / kThreads);
     hipLaunchKernelGGL(k_item_counts, dim3(kblocks), dim3(kThreads), 0, st, ss.bin_start, ss.bin_end, ss.num_keys,
                        ss.chunk, ss.item_cnt);

@@ -652,6 +652,43 @@ def test_time_kernels_advances_like_step(oracle_mod):
     assert all(ms >= 0 for ms, _ in t.values())
 
 
+def test_ticks_binning_after_their_first_p2g(oracle_mod):
+    """Fused ticks after the first run their first P2G on the bins of the tick before and bin after it,
+    gathering every component but v and C (the tick's last G2P rewrites those).  Five ticks of 8 substeps
+    with mid-tick binnings, readback between them, against the FP64 oracle at 40 substeps, at the error
+    level of the tiled engine (which bins before every P2G).  The eager timer shows the order: one P2G
+    per segment, one binning per segment."""
+    p = small_params(substeps=8, mode=abi.MODE_FUSED, rho=4.0)
+    p.sort_interval = 4
+    rng = np.random.default_rng(21)
+    x = rng.uniform(0.3, 0.6, (3000, 2)).astype(np.float32)
+    v = rng.uniform(-1, 1, (3000, 2)).astype(np.float32)
+    mat = rng.integers(0, 3, 3000).astype(np.uint8)
+    with Engine(p) as e:
+        e.set_particles(x, v, material=mat)
+        for _ in range(5):
+            e.step(1)
+            e.sync()
+            out = e.readback()
+        out["grid"] = e.read_grid()
+    s, grid = run_oracle(oracle_mod, p, x, v, mat, 5 * p.substeps)
+    pt = p.copy()
+    pt.mode = abi.MODE_TILED
+    ref = run_gpu(pt, x, v, mat, ticks=5)
+    errs = {k: (rel(out[k], getattr(s, k)), rel(ref[k], getattr(s, k))) for k in ("x", "v", "C", "F", "J")}
+    print(f"[binning after the first P2G, 5 ticks of 8 substeps] (fused, tiled) errors: {errs}", flush=True)
+    assert errs["x"][0] < 1e-6 and errs["F"][0] < 1e-5 and errs["J"][0] < 1e-5
+    assert rel(out["grid"][0][0], grid[0]) < 1e-4
+    # velocities relax from rms 0.58 to about 0.04 in 40 substeps, so their relative rounding error is larger
+    for k in ("v", "C"):
+        assert errs[k][0] < 1e-3 and errs[k][0] < 4 * errs[k][1] + 1e-6, k
+    with Engine(p) as e:
+        e.set_particles(x, v, material=mat)
+        e.step(1)
+        kt = e.time_kernels(2 * p.substeps)
+    assert kt["p2g"][1] == 4 and kt["binning"][1] == 4 and kt["g2p2g"][1] == 4 * 3
+
+
 def test_new_population_on_a_used_handle():
     """mpm_set_particles on a handle that has already run ticks replaces the population completely: the next
     tick equals a fresh handle's on the new particles (a leftover accumulator of the old scene would add its
