@@ -284,9 +284,16 @@ hipError_t launch_g2p_reference(hipStream_t st, const DevParams& p, const Store&
 
 // Sort key of every particle from its position: bin key (the scene comes from the previous key) above the
 // stencil base node within the bin.
-__global__ __launch_bounds__(kThreads) void k_bin_keys(DevParams p, Store s, const int32_t* count, uint32_t* keys)
+// Also clears the bin ranges of the binning (bins without particles keep start = end = 0), which k_bin_bounds
+// fills in later on the same stream.
+__global__ __launch_bounds__(kThreads) void k_bin_keys(DevParams p, Store s, const int32_t* count, uint32_t* keys,
+                                                     int32_t* bstart, int32_t* bend, int nkeys)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    for (int k = i; k < nkeys; k += gridDim.x * blockDim.x) {
+        bstart[k] = 0;
+        bend[k] = 0;
+    }
     if (i >= count[0]) { return; }
     const uint32_t scene = p.num_scenes > 1 ? s.key[i] / (uint32_t)p.bps : 0u;
     uint32_t node;
@@ -705,7 +712,8 @@ hipError_t launch_binning(hipStream_t st, const DevParams& p, const Store& src, 
 {
     const int cap = ss.capacity;
     const unsigned pblocks = (unsigned)((cap + kThreads - 1) / kThreads);
-    hipLaunchKernelGGL(k_bin_keys, dim3(pblocks), dim3(kThreads), 0, st, p, src, count, ss.keys_tmp);
+    hipLaunchKernelGGL(k_bin_keys, dim3(pblocks), dim3(kThreads), 0, st, p, src, count, ss.keys_tmp, ss.bin_start,
+                       ss.bin_end, ss.num_keys);
     // the passes sort by the bin bits only: k_layer_bins ranks the nodes inside each bin itself
     const int bin_bits = ss.key_bits - p.node_bits;
     const int passes = (bin_bits + kRadixBits - 1) / kRadixBits;
@@ -727,10 +735,6 @@ hipError_t launch_binning(hipStream_t st, const DevParams& p, const Store& src, 
     }
     // kin / vin: keys sorted stably by bin and the permutation; bin ranges, then the layered order
     if (layer_bins_lds_bytes(p.node_bits) > 65536) { return hipErrorInvalidValue; }
-    hipError_t e = launch_fill_i32(st, ss.bin_start, (size_t)ss.num_keys, 0);
-    if (e != hipSuccess) { return e; }
-    e = launch_fill_i32(st, ss.bin_end, (size_t)ss.num_keys, 0);
-    if (e != hipSuccess) { return e; }
     hipLaunchKernelGGL(k_bin_bounds, dim3(pblocks), dim3(kThreads), 0, st, kin, count, ss.num_keys, p.node_bits,
                        ss.bin_start, ss.bin_end);
     hipLaunchKernelGGL(k_layer_bins, dim3((unsigned)ss.num_keys), dim3(kThreads), layer_bins_lds_bytes(p.node_bits), st,
@@ -746,7 +750,7 @@ hipError_t launch_binning(hipStream_t st, const DevParams& p, const Store& src, 
     const unsigned kblocks = (unsigned)((ss.num_keys + kThreads - 1) / kThreads);
     hipLaunchKernelGGL(k_item_counts, dim3(kblocks), dim3(kThreads), 0, st, ss.bin_start, ss.bin_end, ss.num_keys,
                        ss.chunk, ss.item_cnt);
-    e = exclusive_scan(st, ss.item_cnt, ss.num_keys, ss.partials);
+    hipError_t e = exclusive_scan(st, ss.item_cnt, ss.num_keys, ss.partials);
     if (e != hipSuccess) { return e; }
     hipLaunchKernelGGL(k_items_write, dim3(kblocks), dim3(kThreads), 0, st, ss.bin_start, ss.bin_end, ss.item_cnt,
                        ss.num_keys, ss.chunk, ss.items, ss.num_items);
