@@ -318,6 +318,8 @@ def main():
                     help="substeps between binnings (0 = the preset's; above the tick's substeps: every k-th tick)")
     ap.add_argument("--chunk", type=int, default=0, metavar="PARTICLES",
                     help="particles per work item (0 = the preset's)")
+    ap.add_argument("--tile", default="", metavar="BXxBY",
+                    help="bin / tile shape in cells, one of 8x8, 16x8, 16x16, 32x16 (empty = the preset's)")
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="mixed1024",
                     help="mixed1024 is the headline; the others are the remaining configurations")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -337,6 +339,8 @@ def main():
         p.sort_interval = args.sort_interval
     if args.chunk > 0:
         p.chunk = args.chunk
+    if args.tile:
+        p.block_x, p.block_y = (int(v) for v in args.tile.lower().split("x"))
     scene = make_scene(args.scale)
     p.capacity = scene.n
     log(f"scene: {scene.description}")

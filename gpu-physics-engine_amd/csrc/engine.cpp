@@ -722,6 +722,11 @@ int32_t mpm_params_preset(int32_t preset, MpmParams* out)
         p.capacity = 4194304;
         p.sort_interval = 100;  // every tick: as fast as every second tick in ticks 6-25, and it stays at
                                 // 7.5-8.0 ms per tick to tick 60, where every second tick reaches 26.6 ms
+        // 16 x 16 bins of about 2,000 particles, one work item each (mpm_autotune's choice): 6.95-6.99 ms per
+        // tick against 7.23-7.30 ms with 16 x 8 / 2048, and 7.56 against 8.17 ms at tick 50
+        p.block_x = 16;
+        p.block_y = 16;
+        p.chunk = 4096;
         break;
     case MPM_PRESET_BATCH64X256:
         unit_square(256, 2e-5f, 16.f);

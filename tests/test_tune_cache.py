@@ -48,4 +48,4 @@ def test_cached_choice_needs_no_engine(tmp_path):
     tuned, res = autotune(p, sc, cache=path)  # a miss would need a GPU; a hit returns at once
     assert (tuned.block_x, tuned.block_y, tuned.chunk, tuned.sort_interval) == (32, 16, 4096, 50)
     assert res.candidates == 0
-    assert (p.block_x, p.block_y) == (16, 8)  # the caller's params are left alone
+    assert (p.block_x, p.block_y, p.chunk) == (16, 16, 4096)  # the caller's params (the preset's) are left alone
