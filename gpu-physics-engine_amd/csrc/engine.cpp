@@ -752,7 +752,11 @@ int32_t mpm_params_preset(int32_t preset, MpmParams* out)
         p.num_scenes = 16;
         p.capacity = 16 * 16777216 / 2;
         p.sort_interval = 100;
-        p.chunk = 3072;  // steady-state sweep: 417.9 ms per tick at 3072 and 4096 against 429.9 at 2048
+        // 16 x 16 bins of about 4,000 particles (16 per cell), one work item each: 390-394 ms per tick against
+        // 403-404 ms with 16 x 8 / 3072 (the analytic engine on the same scenes: 156.5-157.1 against 161-162)
+        p.block_x = 16;
+        p.block_y = 16;
+        p.chunk = 8192;
         p.closure = MPM_CLOSURE_LEARNED;
         break;
     default:

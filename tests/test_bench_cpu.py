@@ -22,7 +22,7 @@ def test_binning_bytes_follow_the_radix_pass_count():
     # keys 12 + P passes x 20 + bin ranges 4 + layered order 20 + gather (4 + 2 x 33 in the fused mode, whose
     # steady-state binning leaves v and C behind; 4 + 2 x 57 otherwise)
     for preset, bins, passes in ((abi.PRESET_MIXED1024, 65 * 65, 2), (abi.PRESET_BATCH64X256, 64 * 17 * 33, 2),
-                                 (abi.PRESET_LEARNED16X2048, 16 * 129 * 257, 3)):
+                                 (abi.PRESET_LEARNED16X2048, 16 * 129 * 129, 3)):
         p = abi.preset(preset)
         assert p.mode == abi.MODE_FUSED
         assert p.num_scenes * (-(-p.grid_w // p.block_x)) * (-(-p.grid_h // p.block_y)) == bins
