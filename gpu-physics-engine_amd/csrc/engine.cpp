@@ -1140,7 +1140,8 @@ int32_t mpm_autotune(const MpmParams* base, int32_t n, const float* x, const flo
     if (best.sort_interval <= 0 || best.sort_interval > substeps) { best.sort_interval = substeps; }
     float best_ms = 3.4e38f;
     int32_t tried = 0;
-    const int chunks[4] = {512, 1024, 2048, 4096};
+    // 8192: one work item for bins of up to 8192 particles (16 x 16 bins at 16 particles per cell)
+    const int chunks[5] = {512, 1024, 2048, 4096, 8192};
     for (int k = 0; k < kNumShapes; ++k) {
         for (int c : chunks) {
             MpmParams p = best;

@@ -232,7 +232,7 @@ typedef struct MpmTuneResult {
 } MpmTuneResult;
 
 /* Time candidate launch configurations of the binned modes on the caller's scene and return the fastest:
- * every compiled tile shape x chunk in {512, 1024, 2048, 4096} at base->sort_interval, then sort intervals
+ * every compiled tile shape x chunk in {512, 1024, 2048, 4096, 8192} at base->sort_interval, then sort intervals
  * {base, 25, 50, 100} (<= substeps) for the winner.  Each candidate is a temporary handle that runs one
  * warm-up tick and one timed tick of `substeps` substeps (HIP events).  Arguments as mpm_set_particles.
  * No counterpart in the reference. */
