@@ -199,59 +199,42 @@ __device__ __forceinline__ void corotated_tau(const Mat2& F, float mu, float lam
     corotated_tau_r(F, c, s, mu, lambda, t00, t01, t11);
 }
 
-// Signed SVD F = U diag(s0, s1) V^T with U, V rotations: polar F = R S, then one Jacobi rotation
-// diagonalising the symmetric S.  Outputs cos/sin of U and V, and of R (= U V^T).
-__device__ __forceinline__ void svd2(const Mat2& F, float& uc, float& us, float& s0, float& s1, float& vc,
-                                     float& vs, float& rc, float& rs)
-{
-    float c, s;
-    polar_cs(F, c, s);
-    rc = c;
-    rs = s;
-    // S = R^T F, R = [[c, -s], [s, c]]
-    const float S00 = c * F.a00 + s * F.a10;
-    const float S01 = c * F.a01 + s * F.a11;
-    const float S10 = -s * F.a00 + c * F.a10;
-    const float S11 = -s * F.a01 + c * F.a11;
-    const float off = 0.5f * (S01 + S10);
-    float jc = 1.f, js = 0.f;
-    if (off != 0.f) {
-        // t = tan of the Jacobi angle, |t| <= 1; the square-root arguments lie in [1, inf) and [1, 2], so
-        // the hardware square root and reciprocal square root (1 ulp) need no denormal scaling
-        const float tau = 0.5f * (S11 - S00) * __builtin_amdgcn_rcpf(off);
-        const float t = copysignf(__builtin_amdgcn_rcpf(fabsf(tau) + __builtin_amdgcn_sqrtf(1.f + tau * tau)), tau);
-        jc = __builtin_amdgcn_rsqf(1.f + t * t);
-        js = t * jc;
-    }
-    // V = [[jc, js], [-js, jc]]
-    s0 = jc * jc * S00 - 2.f * jc * js * off + js * js * S11;
-    s1 = js * js * S00 + 2.f * jc * js * off + jc * jc * S11;
-    vc = jc;
-    vs = -js;  // V as a rotation: [[vc, -vs], [vs, vc]] = [[jc, js], [-js, jc]]
-    // U = R V: rotation by angle(R) + angle(V)
-    uc = c * vc - s * vs;
-    us = s * vc + c * vs;
-}
-
 // Snow return mapping (Stomakhin et al. 2013): clamp the singular values of the trial elastic F to
-// [1 - theta_c, 1 + theta_s], move the removed volume into Jp, rebuild F = U diag(s') V^T.  (rc, rs) is the
-// polar rotation of the trial F, R = U V^T; the clamped singular values are positive, so it is also the
-// polar rotation of the rebuilt F.
+// [1 - theta_c, 1 + theta_s], move the removed volume into Jp, rebuild F with the clamped values.
+// The 2x2 SVD in closed form: every 2x2 matrix is a scaled rotation plus a scaled reflection,
+//   F = [[E, -H], [H, E]] + [[D, G], [G, -D]],  2E = F00 + F11, 2H = F10 - F01, 2D = F00 - F11, 2G = F10 + F01,
+// and with Q = |(E, H)|, R = |(D, G)| its signed SVD F = U diag(s0, s1) V^T (U, V rotations) has s0 = Q + R,
+// s1 = Q - R (negative iff det F < 0), U V^T = Rot, the rotation by (E, H) / Q (the polar rotation of F),
+// and U diag(1, -1) V^T = Refl, the reflection [[c, s], [s, -c]] with (c, s) = (D, G) / R.  Hence
+// U diag(n0, n1) V^T = (n0 + n1)/2 Rot + (n0 - n1)/2 Refl: neither U nor V is formed.  (rc, rs) is the polar
+// rotation from polar_cs (FP64 for det F outside [0.1, 10]); the clamped singular values are positive, so it is
+// also the polar rotation of the rebuilt F.  A non-finite F stays non-finite, as the clamp alone would not keep.
 __device__ __forceinline__ void snow_project(const DevParams& p, Mat2& F, float& Jp, float& rc, float& rs)
 {
-    float uc, us, s0, s1, vc, vs;
-    svd2(F, uc, us, s0, s1, vc, vs, rc, rs);
+    polar_cs(F, rc, rs);
+    const float q = 0.5f * ((F.a00 + F.a11) * rc + (F.a10 - F.a01) * rs);  // Q = (E, H) . (rc, rs)
+    const float d2 = F.a00 - F.a11, g2 = F.a10 + F.a01;
+    const float r2 = d2 * d2 + g2 * g2;  // (2R)^2
+    // below (2R)^2 = 4e-30 F is a scaled rotation to within 1e-15 and the reflection part is dropped; above it
+    // r2 is a normal number and the hardware reciprocal square root (1 ulp) needs no denormal scaling
+    float fc = 1.f, fs = 0.f, r = 0.f;
+    if (r2 > 4e-30f) {
+        const float inv = __builtin_amdgcn_rsqf(r2);
+        fc = d2 * inv;
+        fs = g2 * inv;
+        r = 0.5f * r2 * inv;
+    }
+    const float s0 = q + r, s1 = q - r;
     const float lo = 1.f - p.theta_c, hi = 1.f + p.theta_s;
     const float n0 = fminf(fmaxf(s0, lo), hi);
     const float n1 = fminf(fmaxf(s1, lo), hi);
     Jp *= (s0 * s1) * __builtin_amdgcn_rcpf(n0 * n1);  // n0 n1 in [lo^2, hi^2], near 1
-    // U = [[uc, -us], [us, uc]], V = [[vc, -vs], [vs, vc]]; F = U diag(n) V^T
-    const float u00 = uc, u01 = -us, u10 = us, u11 = uc;
-    const float v00 = vc, v01 = -vs, v10 = vs, v11 = vc;
-    F.a00 = u00 * n0 * v00 + u01 * n1 * v01;
-    F.a01 = u00 * n0 * v10 + u01 * n1 * v11;
-    F.a10 = u10 * n0 * v00 + u11 * n1 * v01;
-    F.a11 = u10 * n0 * v10 + u11 * n1 * v11;
+    const float kr = fmaf(0.f, q, 0.5f * (n0 + n1));  // 0 q: NaN or inf in F reaches the rebuilt F
+    const float kf = 0.5f * (n0 - n1);
+    F.a00 = kr * rc + kf * fc;
+    F.a01 = kf * fs - kr * rs;
+    F.a10 = kr * rs + kf * fs;
+    F.a11 = kr * rc - kf * fc;
 }
 
 // Tait pressure of the tracked J (MPM.cu:68-69): p = k ((1/J)^gamma - 1) clamped to [p_min, p_max].  1/J is

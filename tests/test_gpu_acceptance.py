@@ -58,7 +58,8 @@ def test_mixed1024_full_size_against_oracle(oracle_mod):
     velocities after 100 substeps by 2.8e-5 relative (fluid 1.5e-5, jelly 3.5e-5, snow 3.8e-5), because the
     random initial velocities (rms 0.58) relax to an rms of 0.03 within the tick, so a rounding of the
     starting state is twenty times larger relative to the final velocity; 100 substeps of FP32 rounding
-    measured 9.8e-4 relative in v (positions 2.4e-7).  An error relative to the initial velocity scale is
+    measured 6.4e-4 relative in v (positions 2.4e-7; 9.8e-4 in v with a polar + Jacobi SVD in the snow return
+    mapping instead of the closed form).  An error relative to the initial velocity scale is
     bounded too (1e-4)."""
     p = abi.preset(abi.PRESET_MIXED1024)
     sc = scenes.mixed()
