@@ -121,7 +121,8 @@ __device__ __forceinline__ void bspline(float x, float inv_dx, float halo, int& 
     w[2] = 0.5f * r * r;
 }
 
-__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+// v clamped to [lo, hi], lo <= hi: one v_max_i32 and one v_min_i32 (the nested select form compiles to three)
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
 
 // ---- 2x2 helpers (det2D / matMul2D, cuda-internal.cuh:44-56) ----
 __device__ __forceinline__ float det2(const Mat2& m) { return m.a00 * m.a11 - m.a01 * m.a10; }
