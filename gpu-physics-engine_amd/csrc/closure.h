@@ -26,10 +26,18 @@ constexpr float kSigLo = 0.6f, kSigHi = 1.6f, kJhLo = 0.7f, kJhHi = 1.3f;
 
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float floatx16;
 
-// The packed network computes every hidden pre-activation in the base-2 scale s = -log2(e) z; then
-// silu(z) = z / (1 + e^-z) = -ln 2 * s / (1 + 2^s).  The factor -ln 2 is folded into the next layer:
-// W2 (-ln 2)(-log2 e) = W2 and W3 (-ln 2) (see k_pack_closure), so the activation s * rcp(1 + exp2(s)) is
-// two VALU operations and the hardware exp2 and reciprocal (chain_operands).
+// Hidden activations.  The packed network computes every hidden pre-activation z in index units
+// u = kActScale z + kActShift (W1, W2 scaled by 16, a power of two, so their BF16 parts are those of the
+// unscaled weights; biases 16 b + 160).  SiLU is then piecewise linear over 256 segments of 1/16 in z:
+// segment i = clamp(floor(u), 0, 255) holds a line silu ~ slope_i u + icpt_i, read from an LDS table, so an
+// activation costs a clamp, a conversion, a shift, one LDS read and one FMA instead of the two quarter-rate
+// transcendentals of z / (1 + e^-z).  Segments 1..255 cover z in [-10 + 1/16, 6) with the secant shifted by
+// half its midpoint deviation (|error| <= 1.3e-4), segment 255 extends linearly above z = 6, and segment 0
+// is the constant silu(-10 + 1/16) / 2 for every z below (|error| <= 2.4e-4).  The lines are computed in
+// FP64 on the host (mpm_closure_act_table).
+constexpr int kActSegs = 256;
+constexpr float kActScale = 16.f;
+constexpr float kActShift = 160.f;  // = -16 * (-10): z = -10 is u = 0
 
 // atan2(y, x): atan on [0, 1] from the degree-9 odd polynomial of Abramowitz & Stegun 4.4.47 (|error| <=
 // 1e-5 rad) plus octant reduction.  Training and inference use this same function, so the network's angle
@@ -85,13 +93,14 @@ __host__ __device__ constexpr int chain_unit(int ks, int h, int j)
 // Network operands in MFMA layout (v_mfma_f32_32x32x16_bf16: A lane l holds A[l%32][8(l/32)+j], B lane l
 // holds B[8(l/32)+j][l%32], C lane l holds C[8(r/4)+4(l/32)+r%4][l%32]; checked on the device).
 struct ClosurePacked {
-    // with c = -log2(e), W1s = c W1 split as W1s = W1hi + W1lo (BF16 parts):
+    float2 act[kActSegs];  // SiLU segments (slope, intercept) in index units u
+    // with c = kActScale, W1s = c W1 split as W1s = W1hi + W1lo (BF16 parts):
     bf16x8 a1[2 * 64];  // layer 1, M-tile mt: A[u][8h+j] = (h ? W1lo : W1hi)[u][j % 4], u = 32 mt + l%32
     bf16x8 a2[8 * 64];  // layer 2, (mt, ks): A[m][8h+j] = W2[32 mt + l%32][chain_unit(ks, h, j)]
-    bf16x8 a3[4 * 8];   // layer 3, ks: [8 ks + 4 h + m] = A[m][8h+j] = -ln 2 W3[m][chain_unit(ks, h, j)] for the
+    bf16x8 a3[4 * 8];   // layer 3, ks: [8 ks + 4 h + m] = A[m][8h+j] = W3[m][chain_unit(ks, h, j)] for the
                         // three output rows m < 3, zero for m = 3; rows 3..31 of A are zero and read entry m = 3
-    float b1p[2][32];   // [h][16 mt + r] = c b1[32 mt + 8 (r/4) + 4 h + r%4]
-    float b2p[2][32];   // likewise c b2
+    float b1p[2][32];   // [h][16 mt + r] = c b1[32 mt + 8 (r/4) + 4 h + r%4] + kActShift
+    float b2p[2][32];   // likewise c b2 + kActShift
     float b3[4];
 };
 
@@ -123,19 +132,18 @@ __device__ __forceinline__ floatx16 bias_acc(const float (*bias)[32], int half, 
     return acc;
 }
 
-// Layer output -> next B operand: SiLU and BF16, for K-steps 2 mt and 2 mt + 1.  The 16 activations are
-// computed stage by stage (all exp2, then all reciprocals) so their transcendental latencies overlap.
-__device__ __forceinline__ void chain_operands(const floatx16& acc, bf16x8& lo, bf16x8& hi)
+// Layer output -> next B operand: SiLU and BF16, for K-steps 2 mt and 2 mt + 1.  All 16 table reads are
+// issued before the first FMA so their LDS latencies overlap.  u is finite (the features of every lane are),
+// so the clamped conversion is an index in [0, 255].
+__device__ __forceinline__ void chain_operands(const floatx16& acc, const float2* act, bf16x8& lo, bf16x8& hi)
 {
-    float e[16];
+    float2 c[16];
 #pragma unroll
-    for (int j = 0; j < 16; ++j) { e[j] = __builtin_amdgcn_exp2f(acc[j]); }
-#pragma unroll
-    for (int j = 0; j < 16; ++j) { e[j] = __builtin_amdgcn_rcpf(1.f + e[j]); }
+    for (int j = 0; j < 16; ++j) { c[j] = act[(uint32_t)__builtin_amdgcn_fmed3f(acc[j], 0.f, 255.f)]; }
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-        lo[j] = (__bf16)(acc[j] * e[j]);
-        hi[j] = (__bf16)(acc[8 + j] * e[8 + j]);
+        lo[j] = (__bf16)fmaf(c[j].x, acc[j], c[j].y);
+        hi[j] = (__bf16)fmaf(c[8 + j].x, acc[8 + j], c[8 + j].y);
     }
 }
 
@@ -168,7 +176,7 @@ __device__ __forceinline__ void closure_eval_wave(const ClosurePacked* s, const 
             closure_stage_fence();
             floatx16 acc = bias_acc(s->b1p, half, mt);
             acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(s->a1[mt * 64 + lane], b1, acc, 0, 0, 0);
-            chain_operands(acc, b2[2 * mt], b2[2 * mt + 1]);
+            chain_operands(acc, s->act, b2[2 * mt], b2[2 * mt + 1]);
         }
         bf16x8 b3[4];
 #pragma unroll
@@ -179,7 +187,7 @@ __device__ __forceinline__ void closure_eval_wave(const ClosurePacked* s, const 
             for (int ks = 0; ks < 4; ++ks) {
                 acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(s->a2[(mt * 4 + ks) * 64 + lane], b2[ks], acc, 0, 0, 0);
             }
-            chain_operands(acc, b3[2 * mt], b3[2 * mt + 1]);
+            chain_operands(acc, s->act, b3[2 * mt], b3[2 * mt + 1]);
         }
         closure_stage_fence();
         floatx16 out = {};

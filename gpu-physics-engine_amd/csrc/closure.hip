@@ -232,8 +232,7 @@ __global__ void k_pack_closure(const float* w, ClosurePacked* out)
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = e % 64, m = lane % 32, h = lane / 32;
     bf16x8 v;
-    constexpr float kC = -1.44269504088896341f;  // -log2(e): hidden pre-activations in base-2 scale
-    constexpr float kL = -0.69314718055994531f;  // -ln(2): folded into the output layer
+    constexpr float kC = kActScale;  // hidden pre-activations in index units of the SiLU table
     if (e < 2 * 64) {
         const int u = 32 * (e / 64) + m;
         for (int j = 0; j < 8; ++j) {
@@ -244,23 +243,48 @@ __global__ void k_pack_closure(const float* w, ClosurePacked* out)
         out->a1[e] = v;
     } else if (e < 10 * 64) {
         const int blk = e / 64 - 2, mt = blk / 4, ks = blk % 4;
-        for (int j = 0; j < 8; ++j) { v[j] = (__bf16)w[kOffW2 + kHid * (32 * mt + m) + chain_unit(ks, h, j)]; }
+        for (int j = 0; j < 8; ++j) { v[j] = (__bf16)(kC * w[kOffW2 + kHid * (32 * mt + m) + chain_unit(ks, h, j)]); }
         out->a2[e - 2 * 64] = v;
     } else if (e < 10 * 64 + 32) {
         const int q = e - 10 * 64, ks = q / 8, hq = (q / 4) % 2, mq = q % 4;
-        for (int j = 0; j < 8; ++j) {
-            v[j] = (__bf16)(mq < 3 ? kL * w[kOffW3 + kHid * mq + chain_unit(ks, hq, j)] : 0.f);
-        }
+        for (int j = 0; j < 8; ++j) { v[j] = (__bf16)(mq < 3 ? w[kOffW3 + kHid * mq + chain_unit(ks, hq, j)] : 0.f); }
         out->a3[q] = v;
     } else if (e < 10 * 64 + 32 + 128) {
         // bias of C register r of M-tile mt in half-wave hh
         const int b = e - (10 * 64 + 32), layer = b / 64, hh = (b / 32) % 2, mt = (b / 16) % 2, r = b % 16;
-        const float bias = kC * w[(layer ? kOffB2 : kOffB1) + 32 * mt + 8 * (r / 4) + 4 * hh + r % 4];
+        const float bias = fmaf(kC, w[(layer ? kOffB2 : kOffB1) + 32 * mt + 8 * (r / 4) + 4 * hh + r % 4], kActShift);
         (layer ? out->b2p : out->b1p)[hh][16 * mt + r] = bias;
     } else if (e < 10 * 64 + 32 + 132) {
         const int o = e - (10 * 64 + 32) - 128;
         out->b3[o] = o < 3 ? w[kOffB3 + o] : 0.f;
     }
+}
+
+// SiLU segments in index units u = 16 z + 160 (see closure.h), computed once in FP64:
+// segment i >= 1 covers z in [a, a + 1/16), a = -10 + i/16: the secant of silu over it, shifted by half its
+// deviation at the midpoint (silu is within 1/256 of a parabola there, so the error is halved at the ends and
+// the middle alike); segment 0 is the constant half of silu(-10 + 1/16), the middle of silu's range below.
+static const float* act_table()
+{
+    static const std::vector<float> t = [] {
+        std::vector<float> v(2 * kActSegs);
+        const auto silu = [](double z) { return z / (1.0 + std::exp(-z)); };
+        const double h = 1.0 / (double)kActScale, lo = -(double)kActShift / (double)kActScale;
+        for (int i = 0; i < kActSegs; ++i) {
+            double slope_z = 0.0, icpt_z = 0.5 * silu(lo + h);
+            if (i > 0) {
+                const double a = lo + i * h, b = a + h, mid = a + 0.5 * h;
+                slope_z = (silu(b) - silu(a)) / h;
+                icpt_z = silu(a) - slope_z * a;
+                icpt_z += 0.5 * (silu(mid) - (slope_z * mid + icpt_z));
+            }
+            // z = (u - shift) / scale: silu ~ slope_z z + icpt_z = (slope_z / scale) u + icpt_z - slope_z shift / scale
+            v[2 * i] = (float)(slope_z / kActScale);
+            v[2 * i + 1] = (float)(icpt_z - slope_z * kActShift / kActScale);
+        }
+        return v;
+    }();
+    return t.data();
 }
 
 // Network output of n feature vectors through the wave-level BF16 MFMA path used by the fused kernels.
@@ -272,6 +296,7 @@ __global__ __launch_bounds__(kThreads) void k_closure_eval(ClosureDev c, int n, 
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int ic = i < n ? i : n - 1;
     float x[4] = {X[4 * ic], X[4 * ic + 1], X[4 * ic + 2], X[4 * ic + 3]};
+    for (int k = 0; k < 4; ++k) { x[k] = isfinite(x[k]) ? x[k] : 0.f; }  // the table index needs finite inputs
     float y[3];
     closure_eval_wave(&lds, x, y);
     if (i < n) {
@@ -283,9 +308,13 @@ size_t closure_packed_bytes() { return sizeof(ClosurePacked); }
 
 hipError_t launch_pack_closure(hipStream_t st, const float* w, void* out)
 {
+    ClosurePacked* d = reinterpret_cast<ClosurePacked*>(out);
     hipLaunchKernelGGL(k_pack_closure, dim3((10 * 64 + 32 + 132 + kThreads - 1) / kThreads), dim3(kThreads), 0, st, w,
-                       reinterpret_cast<ClosurePacked*>(out));
-    return hipGetLastError();
+                       d);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) { return e; }
+    // the table is a static host array, so the asynchronous copy may read it at any later time
+    return hipMemcpyAsync(d->act, act_table(), sizeof(d->act), hipMemcpyHostToDevice, st);
 }
 
 }  // namespace mpm
@@ -478,6 +507,13 @@ int32_t mpm_closure_train(int32_t n, const float* features, const float* targets
         *final_loss = tot / (float)batch;
     }
     CL_TRY(hipMemcpy(weights, w, kNumWeights * sizeof(float), hipMemcpyDeviceToHost));
+    return MPM_OK;
+}
+
+int32_t mpm_closure_act_table(float* slope_icpt)
+{
+    if (!slope_icpt) { return MPM_ERR_INVALID_ARG; }
+    std::memcpy(slope_icpt, act_table(), 2 * kActSegs * sizeof(float));
     return MPM_OK;
 }
 

@@ -1252,6 +1252,7 @@ __device__ __forceinline__ Mat2 affine_q_learned(const DevParams& p, const Closu
     float x[4] = {0.f, 0.f, 0.f, 0.f};
     bool use = false;
     if (mat != MPM_MAT_FLUID) { use = closure_features(F, mat == MPM_MAT_SNOW ? J : 1.f, x); }
+    if (!use) { x[0] = x[1] = x[2] = x[3] = 0.f; }  // finite inputs in every lane: the network's table index
     float y[3] = {0.f, 0.f, 0.f};
     if (__ballot(use) != 0ull) { closure_eval_wave(cl, x, y); }
     if (use) {

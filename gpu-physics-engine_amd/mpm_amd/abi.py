@@ -159,6 +159,7 @@ SIGNATURES = {
                                            ctypes.c_uint64, _f32p, _f32p]),
     "mpm_closure_grad": (ctypes.c_int32, [ctypes.c_int32, _f32p, _f32p, _f32p, _f32p, _f32p]),
     "mpm_closure_eval": (ctypes.c_int32, [ctypes.c_int32, _f32p, _f32p, _f32p]),
+    "mpm_closure_act_table": (ctypes.c_int32, [_f32p]),
     "mpm_closure_message": (ctypes.c_char_p, []),
     "mpm_set_closure": (ctypes.c_int32, [_vp, _f32p]),
     "mpm_last_error": (ctypes.c_int32, [_vp]),

@@ -19,28 +19,25 @@ def bf16(a):
     return u.astype(np.uint32).view(np.float32)
 
 
-def silu(z):
-    return z / (1.0 + np.exp(-z))
-
-
 def forward_bf16(X, w):
-    """Restatement of the device inference: hidden pre-activations in base-2 scale s = -log2(e) z, so that
-    silu(z) = -ln 2 * s / (1 + 2^s) with -ln 2 folded into the next layer's weights; layer 1 with operands
-    split into BF16 high and low parts (FP32 accurate), BF16 operands of layers 2 and 3 with FP32
+    """Restatement of the device inference: hidden pre-activations in index units u = 16 z + 160 (weights and
+    biases scaled by 16, biases shifted by 160), SiLU from the engine's piecewise-linear table; layer 1 with
+    operands split into BF16 high and low parts (FP32 accurate), BF16 operands of layers 2 and 3 with FP32
     accumulation and FP32 biases."""
     P = closure.unflatten(w)
-    c = np.float32(-np.log2(np.e))
-    ln2 = np.float32(-np.log(2.0))
+    c = np.float32(closure.ACT_SCALE)
+    shift = np.float32(closure.ACT_SHIFT)
+    table = closure.act_table()
 
     def split(a):
         hi = bf16(a)
         return hi.astype(np.float64) + bf16(a - hi).astype(np.float64)
 
-    def act(s):
-        return s / (1.0 + np.exp2(s))
-    s1 = split(X) @ split(c * P["W1"]).T + (c * P["b1"]).astype(np.float64)
-    s2 = bf16(act(s1)).astype(np.float64) @ bf16(P["W2"]).astype(np.float64).T + (c * P["b2"]).astype(np.float64)
-    return bf16(act(s2)).astype(np.float64) @ bf16(ln2 * P["W3"]).astype(np.float64).T + P["b3"]
+    def bias(b):
+        return (c * b.astype(np.float32) + shift).astype(np.float64)
+    u1 = split(X) @ split(c * P["W1"]).T + bias(P["b1"])
+    u2 = bf16(closure.act_silu(u1, table)).astype(np.float64) @ bf16(c * P["W2"]).astype(np.float64).T + bias(P["b2"])
+    return bf16(closure.act_silu(u2, table)).astype(np.float64) @ bf16(P["W3"]).astype(np.float64).T + P["b3"]
 
 
 def test_gradient_matches_fp64_oracle(oracle_mod):
