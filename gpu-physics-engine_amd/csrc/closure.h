@@ -28,16 +28,18 @@ typedef __attribute__((__vector_size__(16 * sizeof(float)))) float floatx16;
 
 // Hidden activations.  The packed network computes every hidden pre-activation z in index units
 // u = kActScale z + kActShift (W1, W2 scaled by 16, a power of two, so their BF16 parts are those of the
-// unscaled weights; biases 16 b + 160).  SiLU is then piecewise linear over 256 segments of 1/16 in z:
-// segment i = clamp(floor(u), 0, 255) holds a line silu ~ slope_i u + icpt_i, read from an LDS table, so an
-// activation costs a clamp, a conversion, a shift, one LDS read and one FMA instead of the two quarter-rate
-// transcendentals of z / (1 + e^-z).  Segments 1..255 cover z in [-10 + 1/16, 6) with the secant shifted by
-// half its midpoint deviation (|error| <= 1.3e-4), segment 255 extends linearly above z = 6, and segment 0
-// is the constant silu(-10 + 1/16) / 2 for every z below (|error| <= 2.4e-4).  The lines are computed in
-// FP64 on the host (mpm_closure_act_table).
+// unscaled weights; biases 16 b + 159.5).  SiLU is then piecewise linear over 256 segments of 1/16 in z:
+// segment i = clamp(rint(u), 0, 255), which covers z in [-10 + i/16, -10 + (i + 1)/16), holds a line
+// silu ~ slope_i u + icpt_i read from an LDS table.  The index is one v_cvt_pk_u8_f32 (round to nearest even,
+// saturated to [0, 255]), so an activation costs that conversion, a shift, one LDS read and one FMA instead of
+// the two transcendentals of z / (1 + e^-z).  Segments 1..255 cover z in [-10 + 1/16, 6) with the secant
+// shifted by half its midpoint deviation (|error| <= 1.3e-4), segment 255 extends linearly above z = 6, and
+// segment 0 is the constant silu(-10 + 1/16) / 2 for every z below (|error| <= 2.4e-4).  The lines are
+// computed in FP64 on the host (mpm_closure_act_table).
 constexpr int kActSegs = 256;
 constexpr float kActScale = 16.f;
-constexpr float kActShift = 160.f;  // = -16 * (-10): z = -10 is u = 0
+constexpr float kActLo = -10.f;                            // z at the start of segment 0
+constexpr float kActShift = -kActLo * kActScale - 0.5f;    // 159.5: rint(u) = floor(16 (z + 10))
 
 // atan2(y, x): atan on [0, 1] from the degree-9 odd polynomial of Abramowitz & Stegun 4.4.47 (|error| <=
 // 1e-5 rad) plus octant reduction.  Training and inference use this same function, so the network's angle
@@ -133,13 +135,13 @@ __device__ __forceinline__ floatx16 bias_acc(const float (*bias)[32], int half, 
 }
 
 // Layer output -> next B operand: SiLU and BF16, for K-steps 2 mt and 2 mt + 1.  All 16 table reads are
-// issued before the first FMA so their LDS latencies overlap.  u is finite (the features of every lane are),
-// so the clamped conversion is an index in [0, 255].
+// issued before the first FMA so their LDS latencies overlap.  The saturating byte conversion keeps every
+// index in [0, 255] (u is finite in every lane: the features of every lane are).
 __device__ __forceinline__ void chain_operands(const floatx16& acc, const float2* act, bf16x8& lo, bf16x8& hi)
 {
     float2 c[16];
 #pragma unroll
-    for (int j = 0; j < 16; ++j) { c[j] = act[(uint32_t)__builtin_amdgcn_fmed3f(acc[j], 0.f, 255.f)]; }
+    for (int j = 0; j < 16; ++j) { c[j] = act[__builtin_amdgcn_cvt_pk_u8_f32(acc[j], 0u, 0u)]; }
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
         lo[j] = (__bf16)fmaf(c[j].x, acc[j], c[j].y);

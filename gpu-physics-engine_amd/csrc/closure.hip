@@ -260,7 +260,7 @@ __global__ void k_pack_closure(const float* w, ClosurePacked* out)
     }
 }
 
-// SiLU segments in index units u = 16 z + 160 (see closure.h), computed once in FP64:
+// SiLU segments in index units u = 16 z + 159.5 (see closure.h), computed once in FP64:
 // segment i >= 1 covers z in [a, a + 1/16), a = -10 + i/16: the secant of silu over it, shifted by half its
 // deviation at the midpoint (silu is within 1/256 of a parabola there, so the error is halved at the ends and
 // the middle alike); segment 0 is the constant half of silu(-10 + 1/16), the middle of silu's range below.
@@ -269,7 +269,7 @@ static const float* act_table()
     static const std::vector<float> t = [] {
         std::vector<float> v(2 * kActSegs);
         const auto silu = [](double z) { return z / (1.0 + std::exp(-z)); };
-        const double h = 1.0 / (double)kActScale, lo = -(double)kActShift / (double)kActScale;
+        const double h = 1.0 / (double)kActScale, lo = (double)kActLo;
         for (int i = 0; i < kActSegs; ++i) {
             double slope_z = 0.0, icpt_z = 0.5 * silu(lo + h);
             if (i > 0) {

@@ -74,13 +74,13 @@ def random_deformations(n, seed, sig_lo=0.6, sig_hi=1.6, jp_lo=0.7, jp_hi=1.3):
     return F.astype(np.float32), J.astype(np.float32), mat
 
 
-ACT_SCALE = 16.0  # hidden pre-activation z in the inference path's index units: u = 16 z + 160
-ACT_SHIFT = 160.0
+ACT_SCALE = 16.0  # hidden pre-activation z in the inference path's index units: u = 16 z + 159.5
+ACT_SHIFT = 159.5
 
 
 def act_table():
-    """The inference path's SiLU segments: (slope[256], intercept[256]) float32 in index units u = 16 z + 160;
-    silu(z) ~ slope[i] u + intercept[i] with i = clamp(floor(u), 0, 255)."""
+    """The inference path's SiLU segments: (slope[256], intercept[256]) float32 in index units u = 16 z + 159.5;
+    silu(z) ~ slope[i] u + intercept[i] with i = clamp(rint(u), 0, 255), rint rounding half to even."""
     t = np.zeros(512, np.float32)
     _lib_check(abi.load_library().mpm_closure_act_table(_p(t)), "mpm_closure_act_table")
     return t[0::2].copy(), t[1::2].copy()
@@ -90,7 +90,7 @@ def act_silu(u, table=None):
     """Piecewise-linear SiLU of index-unit pre-activations u (float64 evaluation of the device's FMA)."""
     slope, icpt = act_table() if table is None else table
     u = np.asarray(u, dtype=np.float64)
-    i = np.clip(np.floor(u), 0, 255).astype(np.int64)
+    i = np.clip(np.rint(u), 0, 255).astype(np.int64)
     return slope[i].astype(np.float64) * u + icpt[i].astype(np.float64)
 
 

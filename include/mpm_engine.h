@@ -217,7 +217,8 @@ int32_t mpm_closure_grad(int32_t n, const float* features, const float* targets,
 /* Network outputs float[3n] of n feature vectors through the BF16 MFMA inference path of the engine. */
 int32_t mpm_closure_eval(int32_t n, const float* features, const float* weights, float* out);
 /* The inference path's SiLU: 256 linear segments (slope, intercept) as float[512], in index units
- * u = 16 z + 160 of the pre-activation z; segment clamp(floor(u), 0, 255) gives silu(z) ~ slope u + intercept.
+ * u = 16 z + 159.5 of the pre-activation z; segment clamp(rint(u), 0, 255) (round half to even) gives
+ * silu(z) ~ slope u + intercept.
  * Host computation (no device needed); the same table is copied into every packed network. */
 int32_t mpm_closure_act_table(float* slope_icpt);
 /* Message of the last failed closure call. */

@@ -1,6 +1,6 @@
 """CPU checks of the learned closure's inference activation: the piecewise-linear SiLU table the engine
 builds on the host (mpm_closure_act_table) against the exact SiLU z / (1 + e^-z), evaluated the way the
-device does (index clamp(floor(u), 0, 255) of u = 16 z + 160, one FMA)."""
+device does (index clamp(rint(u), 0, 255) of u = 16 z + 159.5, one FMA)."""
 import numpy as np
 
 from mpm_amd import closure
@@ -11,7 +11,7 @@ def silu(z):
 
 
 def table_silu(z, table):
-    u = 16.0 * np.asarray(z, dtype=np.float64) + 160.0
+    u = 16.0 * np.asarray(z, dtype=np.float64) + 159.5
     return closure.act_silu(u, table)
 
 
@@ -21,7 +21,8 @@ def test_table_segments_and_error_bound():
     assert slope.shape == (256,) and icpt.shape == (256,)
     assert np.all(np.isfinite(slope)) and np.all(np.isfinite(icpt))
     # interior segments: |error| <= 1.3e-4 (1/256 of a parabola with |silu''| <= 1/2, halved by the shift)
-    z = np.linspace(-10 + 1 / 16, 6, 400001)[:-1]
+    # (open at the left end: at z = -10 + 1/16 exactly, u = 0.5 rounds to segment 0, the constant below)
+    z = np.linspace(-10 + 1 / 16, 6, 400001)[1:-1]
     err = np.abs(table_silu(z, table) - silu(z))
     assert err.max() < 1.3e-4, err.max()
     # the shift halves the secant's error: without it the bound near z = 0 would be 2.4e-4
@@ -38,13 +39,13 @@ def test_table_segments_and_error_bound():
 
 
 def test_table_index_units():
-    """u = 16 z + 160: the segment boundaries are the multiples of 1/16 in z, and the lines of neighbouring
+    """u = 16 z + 159.5: the segment boundaries are the multiples of 1/16 in z, and the lines of neighbouring
     interior segments meet within 1e-5 at the boundaries (the FP32 rounding of the intercepts; BF16 operands
     cannot resolve it)."""
     slope, icpt = closure.act_table()
-    u = np.arange(2, 256, dtype=np.float64)
+    u = np.arange(2, 256, dtype=np.float64) - 0.5  # boundaries: rint switches segment at half-integers
     left = slope[1:255].astype(np.float64) * u + icpt[1:255]
     right = slope[2:256].astype(np.float64) * u + icpt[2:256]
     assert np.abs(left - right).max() < 1e-5
-    z = (u - 160.0) / 16.0
+    z = (u - 159.5) / 16.0
     assert np.abs(right - silu(z)).max() < 1.3e-4

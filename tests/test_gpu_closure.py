@@ -20,8 +20,8 @@ def bf16(a):
 
 
 def forward_bf16(X, w):
-    """Restatement of the device inference: hidden pre-activations in index units u = 16 z + 160 (weights and
-    biases scaled by 16, biases shifted by 160), SiLU from the engine's piecewise-linear table; layer 1 with
+    """Restatement of the device inference: hidden pre-activations in index units u = 16 z + 159.5 (weights and
+    biases scaled by 16, biases shifted by 159.5), SiLU from the engine's piecewise-linear table; layer 1 with
     operands split into BF16 high and low parts (FP32 accurate), BF16 operands of layers 2 and 3 with FP32
     accumulation and FP32 biases."""
     P = closure.unflatten(w)
