@@ -64,9 +64,11 @@ __device__ __forceinline__ float fast_atan2(float y, float x)
 // Closure features of one particle; returns false when outside the trained range (then x is unspecified).
 // The SVD comes in closed form: with E = (F00 + F11) / 2, D = (F00 - F11) / 2, G = (F10 + F01) / 2 and
 // H = (F10 - F01) / 2, F = U diag(s0, s1) V^T with s0 = q + r >= s1 = q - r for q = |(E, H)|, r = |(D, G)|,
-// and U the rotation by (atan2(H, E) + atan2(G, D)) / 2.  tau = U diag U^T is unchanged by U -> -U, so the
-// angle is reduced to (-pi/2, pi/2]; at s0 = s1 any U serves (there tau is isotropic).  In the trained range
-// q >= 0.6 (q is at least the mean singular value), so the hardware square root needs no denormal scaling.
+// and U the rotation by (atan2(H, E) + atan2(G, D)) / 2.  tau = U diag U^T is unchanged by U -> -U, so only
+// that angle mod pi matters: it is half the argument of (E + iH)(D + iG) = (ED - HG) + i(EG + HD), one atan2,
+// and atan2 / 2 lies in (-pi/2, pi/2] already.  At s0 = s1 any U serves (there tau is isotropic).  In the
+// trained range q >= 0.6 (q is at least the mean singular value), so the hardware square root needs no
+// denormal scaling.
 __device__ __forceinline__ bool closure_features(const Mat2& F, float jh, float x[4])
 {
     const float E = 0.5f * (F.a00 + F.a11), D = 0.5f * (F.a00 - F.a11);
@@ -74,9 +76,7 @@ __device__ __forceinline__ bool closure_features(const Mat2& F, float jh, float 
     const float q = __builtin_amdgcn_sqrtf(fmaf(E, E, H * H));
     const float r = __builtin_amdgcn_sqrtf(fmaf(D, D, G * G));
     const float s0 = q + r, s1 = q - r;
-    float th = 0.5f * (fast_atan2(H, E) + fast_atan2(G, D));
-    if (th > 1.57079632679f) { th -= 3.14159265359f; }
-    if (th <= -1.57079632679f) { th += 3.14159265359f; }
+    const float th = 0.5f * fast_atan2(fmaf(E, G, H * D), fmaf(E, D, -H * G));
     x[0] = (s0 - 1.1f) * 2.f;
     x[1] = (s1 - 1.1f) * 2.f;
     x[2] = th * 0.636619772368f;
@@ -157,21 +157,29 @@ __device__ __forceinline__ void closure_eval_wave(const ClosurePacked* s, const 
 {
     const int lane = threadIdx.x & 63;
     const int half = lane >> 5;
+    // layer 1 B operand: (x_hi, x_lo) of particle 32 nt + l%32 in both halves, so that
+    // A1 B1 = W1hi (x_hi + x_lo) + W1lo (x_hi + x_lo).  Each lane splits its own features once; the two
+    // half-wave exchanges of the packed words then give the operands of both N-tiles.
+    bf16x8 own;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const __bf16 fh = (__bf16)x[k];
+        own[k] = fh;
+        own[4 + k] = (__bf16)(x[k] - (float)fh);
+    }
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    const u32x4 ow = __builtin_bit_cast(u32x4, own);
+    u32x4 op[2];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const auto sw = __builtin_amdgcn_permlane32_swap(ow[q], ow[q], false, false);
+        op[0][q] = sw[0];
+        op[1][q] = sw[1];
+    }
     float res[2][3];
 #pragma unroll
     for (int nt = 0; nt < 2; ++nt) {
-        // layer 1 B operand: (x_hi, x_lo) of particle 32 nt + l%32 in both halves, so that
-        // A1 B1 = W1hi (x_hi + x_lo) + W1lo (x_hi + x_lo)
-        bf16x8 b1;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(x[k]), __float_as_uint(x[k]), false,
-                                                             false);
-            const float f = __uint_as_float(nt == 0 ? sw[0] : sw[1]);
-            const __bf16 fh = (__bf16)f;
-            b1[k] = fh;
-            b1[4 + k] = (__bf16)(f - (float)fh);
-        }
+        const bf16x8 b1 = __builtin_bit_cast(bf16x8, op[nt]);
         bf16x8 b2[4];
 #pragma unroll
         for (int mt = 0; mt < 2; ++mt) {
