@@ -797,17 +797,47 @@ __device__ __forceinline__ float2 grid_velocity(const DevParams& p, const Grid& 
     return g.vel[n];
 }
 
+// Two nodes per thread per pass, with all their loads issued before either velocity is formed: a tile of
+// N <= 2 kThreads nodes waits for HBM once, not twice.
 template <int BX, int BY>
 __device__ __forceinline__ void load_vel_tile(const DevParams& p, const Grid& g, size_t gb, int ox, int oy, float2* sv)
 {
     using T = TileShape<BX, BY>;
-    for (int t = threadIdx.x; t < T::N; t += kThreads) {
-        const int ty = t / T::TX;
-        const int tx = t - ty * T::TX;
-        const int gx = ox + tx, gy = oy + ty;
-        float2 v = {0.f, 0.f};
-        if (gx >= 0 && gx < p.W && gy >= 0 && gy < p.H) { v = grid_velocity(p, g, gb, gx, gy); }
-        sv[t] = v;
+    for (int t0 = threadIdx.x; t0 < T::N; t0 += 2 * kThreads) {
+        int gx[2], gy[2];
+        bool in[2];
+        float mx[2], my[2], m[2];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int t = t0 + k * kThreads;
+            const int ty = t / T::TX;
+            const int tx = t - ty * T::TX;
+            gx[k] = ox + tx;
+            gy[k] = oy + ty;
+            in[k] = t < T::N && gx[k] >= 0 && gx[k] < p.W && gy[k] >= 0 && gy[k] < p.H;
+            mx[k] = my[k] = m[k] = 0.f;
+            if (in[k]) {
+                const size_t n = gb + (size_t)gy[k] * p.W + gx[k];
+                if (g.src_m) {
+                    mx[k] = g.src_x[n];
+                    my[k] = g.src_y[n];
+                    m[k] = g.src_m[n];
+                } else {
+                    const float2 v = g.vel[n];
+                    mx[k] = v.x;
+                    my[k] = v.y;
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int t = t0 + k * kThreads;
+            if (t < T::N) {
+                float2 v = {0.f, 0.f};
+                if (in[k]) { v = g.src_m ? node_velocity(p, mx[k], my[k], m[k], gx[k], gy[k]) : make_float2(mx[k], my[k]); }
+                sv[t] = v;
+            }
+        }
     }
 }
 
@@ -1211,24 +1241,43 @@ k_g2p2g(
     zero_acc_tiles<BX, BY>(acc);
     __syncthreads();
     const auto wtile = acc.wave(__builtin_amdgcn_readfirstlane(threadIdx.x >> 6));  // wave-uniform
-    // The position of the lane's next particle is loaded one iteration ahead (the gather needs it first;
-    // F, J and the material are needed later and keep their loads in the iteration).
+    // The state of the lane's next particle is loaded one iteration ahead, so an iteration does not wait for
+    // HBM: all of it when the register budget of the tile shape allows (at most 6 waves per SIMD, 84+ VGPRs),
+    // else the position only (the gather needs it first).
+    constexpr bool kAll = WavesPerSimd<BX, BY, false, 7>::value <= 6;
     int j = item.begin + threadIdx.x;
-    float nx = 0.f, ny = 0.f;
+    float nx = 0.f, ny = 0.f, nJ = 1.f;
+    int nmat = 0;
+    Mat2 nF = {1.f, 0.f, 0.f, 1.f};
     if (j < item.end) {
         nx = ldo(s.x, (uint32_t)j * 4u);
         ny = ldo(s.y, (uint32_t)j * 4u);
+        if (kAll) {
+            nmat = ldo(s.mat, (uint32_t)j);
+            nF = ld_F(s, (uint32_t)j * 4u);
+            nJ = ldo(s.J, (uint32_t)j * 4u);
+        }
     }
     for (; j < item.end; j += kThreads) {
         const uint32_t o = (uint32_t)j * 4u;
         float px = nx, py = ny, vx, vy;
+        int mat = nmat;
+        Mat2 C, F = nF;
+        float J = nJ;
+        if (!kAll) {
+            mat = ldo(s.mat, (uint32_t)j);
+            F = ld_F(s, o);
+            J = ldo(s.J, o);
+        }
         if (j + kThreads < item.end) {
             nx = ldo(s.x, o + 4u * kThreads);
             ny = ldo(s.y, o + 4u * kThreads);
+            if (kAll) {
+                nmat = ldo(s.mat, (uint32_t)(j + kThreads));
+                nF = ld_F(s, o + 4u * kThreads);
+                nJ = ldo(s.J, o + 4u * kThreads);
+            }
         }
-        const int mat = ldo(s.mat, (uint32_t)j);
-        Mat2 C, F = ld_F(s, o);
-        float J = ldo(s.J, o);
         gather<BX, BY>(p, g, gb, ox, oy, sv, px, py, vx, vy, C);
         SnowHint snow = {1.f, 0.f, 1.f};  // a snow particle's rotation and hardening, from its return mapping
         g2p_update(p, mat, px, py, vx, vy, C, F, J, &snow);
