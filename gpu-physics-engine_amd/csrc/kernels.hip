@@ -535,7 +535,11 @@ __global__ __launch_bounds__(kThreads) void k_bin_bounds(const uint32_t* keys, c
 // that drift by a cell during the substeps between two sorts still mostly start at distinct nodes.  The
 // last layer takes the remaining particles in (checkerboard slot, rank) order.  One work group per bin; the
 // result equals the layered order of a stable sort by (bin, node).
-constexpr int kLayers = 16;
+// Most layers of the in-bin order: particle of rank r at its node goes to layer min(r, kLayers - 1).  Scenes of
+// 16 particles per cell put about 16 at a node, so fewer layers left long runs of one node in the last
+// layer (batch64x256: 2.5 scatter rounds per wave with 16 layers).  A bin runs only as many layers as its
+// fullest node needs; the order is the same as with all kLayers.
+constexpr int kLayers = 24;
 
 // Node (ly * bin_x + lx) at checkerboard slot q of a bin of nn = bin_x * bin_y nodes, bin_x even: slots
 // [0, nn/2) are the nodes with lx + ly even, slots [nn/2, nn) those with lx + ly odd, each in node order.
@@ -549,12 +553,12 @@ __device__ __forceinline__ int checker_node(int q, int nn, int bin_x)
     return ly * bin_x + lx;
 }
 // LDS of k_layer_bins for bins of 2^node_bits nodes (at least 64: the node prefixes run over whole waves);
-// 43.6 KB for the largest compiled shape (32 x 16 cells, 512 nodes).
+// 60.3 KB for the largest compiled shape (32 x 16 cells, 512 nodes).
 static size_t layer_bins_lds_bytes(int node_bits)
 {
     const size_t nn = (size_t)1 << node_bits;
     return sizeof(int) * ((size_t)(kThreads / 64) * nn + nn + (size_t)kLayers * nn + (size_t)kLayers * ((nn + 63) / 64) +
-                          (size_t)kLayers);
+                          (size_t)kLayers + 1);
 }
 __global__ __launch_bounds__(kThreads) void k_layer_bins(const uint32_t* keys_in, const int32_t* perm_in,
                                                          const int32_t* bstart, const int32_t* bend, int nkeys,
@@ -569,6 +573,7 @@ __global__ __launch_bounds__(kThreads) void k_layer_bins(const uint32_t* keys_in
     //   pre[l * nn + n]    layer l < last: nodes before n in the layer; last: particles before n
     //   tot[l * nw + v]    per layer and node wave v (nw = nn / 64): the nodes (or particles) it holds
     //   loff[l]            first position of layer l
+    //   nl                 layers this bin uses: min(most particles at one node, kLayers)
     extern __shared__ int lds[];
     const int nn = 1 << node_bits;
     const int nw = (nn + 63) >> 6;
@@ -577,6 +582,7 @@ __global__ __launch_bounds__(kThreads) void k_layer_bins(const uint32_t* keys_in
     int* const pre = cnt + nn;
     int* const tot = pre + kLayers * nn;
     int* const loff = tot + kLayers * nw;
+    int* const nl_lds = loff + kLayers;
     const int k = blockIdx.x;
     if (k >= nkeys) { return; }
     const int s = bstart[k], e = bend[k];
@@ -594,6 +600,7 @@ __global__ __launch_bounds__(kThreads) void k_layer_bins(const uint32_t* keys_in
 #pragma unroll
         for (int w = 0; w < kWaves; ++w) { wrun[w * nn + n] = 0; }
     }
+    if (threadIdx.x == 0) { *nl_lds = 1; }
     __syncthreads();
     for (int j = ws + lane; j < we; j += 64) { atomicAdd(&wrun[wave * nn + (int)(keys_in[j] & mask)], 1); }
     __syncthreads();
@@ -607,35 +614,35 @@ __global__ __launch_bounds__(kThreads) void k_layer_bins(const uint32_t* keys_in
             c += t;
         }
         cnt[n] = c;
+        if (c > 1) { atomicMax(nl_lds, min(c, kLayers)); }
     }
     __syncthreads();
+    const int nl = *nl_lds;  // layers [nl, kLayers) would be empty: the order is that of kLayers layers
     // in-wave prefixes over checkerboard slots by ballot (layers below the last) and by a wave scan (last)
     for (int q = threadIdx.x; q < nn; q += kThreads) {
         const int n = checker_node(q, nn, bin_x);
         const int c = cnt[n];
         const int w = q >> 6;
-#pragma unroll
-        for (int l = 0; l < kLayers - 1; ++l) {
+        for (int l = 0; l < nl - 1; ++l) {
             const uint64_t m = __ballot(c > l);
             pre[l * nn + n] = __popcll(m & lt_mask);
             if (lane == 0) { tot[l * nw + w] = __popcll(m); }
         }
-        const int over = c > kLayers - 1 ? c - (kLayers - 1) : 0;
+        const int over = c > nl - 1 ? c - (nl - 1) : 0;
         int incl = over;
 #pragma unroll
         for (int d = 1; d < 64; d <<= 1) {
             const int t = __shfl_up(incl, d, 64);
             if (lane >= d) { incl += t; }
         }
-        pre[(kLayers - 1) * nn + n] = incl - over;
-        if (lane == 63) { tot[(kLayers - 1) * nw + w] = incl; }
+        pre[(nl - 1) * nn + n] = incl - over;
+        if (lane == 63) { tot[(nl - 1) * nw + w] = incl; }
     }
     __syncthreads();
     for (int q = threadIdx.x; q < nn; q += kThreads) {
         const int n = checker_node(q, nn, bin_x);
         const int w = q >> 6;
-#pragma unroll
-        for (int l = 0; l < kLayers; ++l) {
+        for (int l = 0; l < nl; ++l) {
             int carry = 0;
             for (int v = 0; v < w; ++v) { carry += tot[l * nw + v]; }
             pre[l * nn + n] += carry;
@@ -643,7 +650,7 @@ __global__ __launch_bounds__(kThreads) void k_layer_bins(const uint32_t* keys_in
     }
     if (threadIdx.x == 0) {
         int o = 0;
-        for (int l = 0; l < kLayers; ++l) {
+        for (int l = 0; l < nl; ++l) {
             loff[l] = o;
             for (int v = 0; v < nw; ++v) { o += tot[l * nw + v]; }
         }
@@ -669,8 +676,7 @@ __global__ __launch_bounds__(kThreads) void k_layer_bins(const uint32_t* keys_in
         __asm__ __volatile__("" ::: "memory");
         if (valid && (peers >> lane) == 1ull) { wrun[wave * nn + n] = r + 1; }
         __asm__ __volatile__("" ::: "memory");
-        const int off = r < kLayers - 1 ? loff[r] + pre[r * nn + n]
-                                        : loff[kLayers - 1] + pre[(kLayers - 1) * nn + n] + (r - (kLayers - 1));
+        const int off = r < nl - 1 ? loff[r] + pre[r * nn + n] : loff[nl - 1] + pre[(nl - 1) * nn + n] + (r - (nl - 1));
         if (valid && off < len) {  // always true for a correct ranking; keeps a wrong one inside the bin
             perm_out[s + off] = perm_in[j];
             keys_out[s + off] = key >> node_bits;

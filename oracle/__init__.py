@@ -207,7 +207,7 @@ def bin_slots(params, x, block_x, block_y):
     return checker_slots(bin_nodes(params, x, block_x, block_y), block_x, block_y)
 
 
-LAYERS = 16
+LAYERS = 24
 
 
 def layered_order(keys, nodes, layers=LAYERS):

@@ -453,7 +453,7 @@ def test_layered_order(oracle_mod):
     assert oracle_mod.layered_order(k, n2, layers=2).tolist() == [3, 0, 4, 5, 1, 2]
     rng = np.random.default_rng(12)
     keys = rng.integers(0, 7, 3000).astype(np.uint32)
-    nodes = np.minimum(rng.geometric(0.08, 3000) - 1, 127).astype(np.uint32)  # some nodes above 16 particles
+    nodes = np.minimum(rng.geometric(0.08, 3000) - 1, 127).astype(np.uint32)  # some nodes above 24 particles
     perm = oracle_mod.layered_order(keys, nodes)
     assert np.array_equal(np.sort(perm), np.arange(3000))
     assert np.all(np.diff(keys[perm].astype(np.int64)) >= 0)  # bins contiguous and ascending
@@ -462,11 +462,13 @@ def test_layered_order(oracle_mod):
     for i in range(3000):  # rank in input order
         rank[i] = seen.get((keys[i], nodes[i]), 0)
         seen[(keys[i], nodes[i])] = rank[i] + 1
-    layer = np.minimum(rank, 15)
+    last = oracle_mod.LAYERS - 1
+    assert rank.max() > last  # the last layer holds more than one rank
+    layer = np.minimum(rank, last)
     brute = sorted(range(3000), key=lambda i: (keys[i], layer[i], nodes[i], rank[i]))
     assert perm.tolist() == brute
     for b in range(7):  # every layer below the last holds distinct nodes
-        for l in range(15):
+        for l in range(last):
             sel = (keys == b) & (layer == l)
             assert len(np.unique(nodes[sel])) == sel.sum()
 
