@@ -537,8 +537,9 @@ __global__ __launch_bounds__(kThreads) void k_bin_bounds(const uint32_t* keys, c
 // result equals the layered order of a stable sort by (bin, node).
 // Most layers of the in-bin order: particle of rank r at its node goes to layer min(r, kLayers - 1).  Scenes of
 // 16 particles per cell put about 16 at a node, so fewer layers left long runs of one node in the last
-// layer (batch64x256: 2.5 scatter rounds per wave with 16 layers).  A bin runs only as many layers as its
-// fullest node needs; the order is the same as with all kLayers.
+// layer (64 scenes of 256^2 cells at 16 per cell: 2.5 scatter rounds per wave with 16 layers, 8 % slower
+// ticks).  A bin runs only as many layers as its fullest node needs; the order is the same as with all
+// kLayers.
 constexpr int kLayers = 24;
 
 // Node (ly * bin_x + lx) at checkerboard slot q of a bin of nn = bin_x * bin_y nodes, bin_x even: slots
@@ -768,8 +769,8 @@ hipError_t launch_binning(hipStream_t st, const DevParams& p, const Store& src, 
 // Work groups are dispatched round-robin over the 8 XCDs; in one scene, give each XCD a contiguous run of
 // work items (neighbouring bins) so the velocity tiles and accumulator flushes of neighbours share that
 // XCD's L2.  With several scenes the items go round-robin too: the runs would give whole scenes of unequal
-// cost to one XCD (measured: batch64x256 -2.5 % and learned16x2048 -2 % per tick with the round-robin
-// order, mixed1024 +2.7 % with it).
+// cost to one XCD (measured per tick with the round-robin order: 64 scenes of 256^2 cells -2.5 %, 16 scenes
+// of 2048^2 -2 %, one scene of 1024^2 +2.7 %).
 __device__ __forceinline__ int xcd_item(int num_items, bool round_robin, bool& ok)
 {
     if (round_robin) {
