@@ -1327,7 +1327,9 @@ __device__ __forceinline__ Mat2 affine_q_learned(const DevParams& p, const Closu
     return affine_q(p, mat, C, F, J);
 }
 
-// k_p2g_tiled with the learned closure (uniform trip count so every wave runs the network with 64 lanes).
+// k_p2g_tiled with the learned closure.  Every wave runs the network with all 64 lanes: a wave's trip count
+// is uniform over its lanes (lanes past the item's end ride along invalid), and a wave stops when its
+// first lane passes the end, so no wave evaluates a network for particles it does not have.
 template <int BX, int BY>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WavesPerSimd<BX, BY, true, 5>::value))) void
 k_p2g_tiled_learned(DevParams p, Store s, Grid g, const Item* items,
@@ -1349,9 +1351,11 @@ k_p2g_tiled_learned(DevParams p, Store s, Grid g, const Item* items,
     zero_acc_tiles<BX, BY>(acc);
     closure_stage(p.cl, &cl);
     __syncthreads();
-    const auto wtile = acc.wave(__builtin_amdgcn_readfirstlane(threadIdx.x >> 6));  // wave-uniform
-    for (int base = item.begin; base < item.end; base += kThreads) {
-        const int j = base + threadIdx.x;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
+    const auto wtile = acc.wave(wave);
+    const int lane = threadIdx.x & 63;
+    for (int base = item.begin + 64 * wave; base < item.end; base += kThreads) {
+        const int j = base + lane;
         const bool valid = j < item.end;
         const uint32_t o = (uint32_t)(valid ? j : item.end - 1) * 4u;
         const int mat = ldo(s.mat, o >> 2);
@@ -1363,7 +1367,7 @@ k_p2g_tiled_learned(DevParams p, Store s, Grid g, const Item* items,
 }
 
 // k_g2p2g with the learned closure: the P2G stress of every wave comes from one BF16 MFMA evaluation of
-// the network for its 64 particles.
+// the network for its 64 particles (per-wave trip counts as in k_p2g_tiled_learned).
 template <int BX, int BY>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WavesPerSimd<BX, BY, true, 5>::value))) void
 k_g2p2g_learned(DevParams p, Store s, Grid g, const Item* items,
@@ -1387,9 +1391,11 @@ k_g2p2g_learned(DevParams p, Store s, Grid g, const Item* items,
     zero_acc_tiles<BX, BY>(acc);
     closure_stage(p.cl, &cl);
     __syncthreads();
-    const auto wtile = acc.wave(__builtin_amdgcn_readfirstlane(threadIdx.x >> 6));  // wave-uniform
-    for (int base = item.begin; base < item.end; base += kThreads) {
-        const int j = base + threadIdx.x;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
+    const auto wtile = acc.wave(wave);
+    const int lane = threadIdx.x & 63;
+    for (int base = item.begin + 64 * wave; base < item.end; base += kThreads) {
+        const int j = base + lane;
         const bool valid = j < item.end;
         const uint32_t o = (uint32_t)(valid ? j : item.end - 1) * 4u;
         float px = ldo(s.x, o), py = ldo(s.y, o), vx, vy;
