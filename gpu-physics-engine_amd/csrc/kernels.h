@@ -76,6 +76,9 @@ hipError_t launch_g2p_tiled(hipStream_t st, int shape, const DevParams& p, const
                             const SortScratch& ss, const int32_t* count);
 hipError_t launch_g2p2g(hipStream_t st, int shape, const DevParams& p, const Store& s, const Grid& g,
                         const SortScratch& ss, const int32_t* count);
+// The analytic-closure branch of launch_g2p2g (g2p2g.hip).
+hipError_t launch_g2p2g_plain(hipStream_t st, int shape, const DevParams& p, const Store& s, const Grid& g,
+                              const SortScratch& ss, const int32_t* count);
 
 hipError_t launch_output(hipStream_t st, const Store& s, int32_t* count, int cap);
 // Pack W2 of the closure (float[4675] parameters) into the MFMA A-operand layout (8 x 64 bf16x8).

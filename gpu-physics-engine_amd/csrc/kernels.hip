@@ -12,14 +12,32 @@
 // plus the binning that the reference MPM does not have: a stable radix sort by (bin, stencil base node)
 // and a deterministic layered order inside each bin (the reference's only sort, PBF.cu:738-834, ranks with
 // atomicAdd and is order-nondeterministic).
+//
+// The plain fused kernel k_g2p2g is compiled in a translation unit of its own (g2p2g.hip includes this file
+// with MPM_PLAIN_FUSED_TU defined) so that it alone can take LLVM's max-ilp machine scheduler; everything
+// that must exist once (kernels with external linkage, launchers, device variables) is guarded by
+// MPM_MAIN_TU, the shared device code and the kernel templates are seen by both.
 #include "kernels.h"
 #include "closure.h"
 
 #include <hip/hip_runtime.h>
 
+#ifndef MPM_PLAIN_FUSED_TU
+#define MPM_MAIN_TU 1
+#endif
+
 namespace mpm {
 
 // ===================================================================================== small utilities
+
+__device__ __forceinline__ int scene_of_key(const DevParams& p, uint32_t key)
+{
+    return p.num_scenes > 1 ? (int)(key / (uint32_t)p.bps) : 0;
+}
+
+__device__ __forceinline__ void atomic_add_global(float* a, float v) { unsafeAtomicAdd(a, v); }
+
+#ifdef MPM_MAIN_TU
 
 __global__ void k_fill_f32(float* p, size_t n, float v)
 {
@@ -53,13 +71,6 @@ hipError_t launch_fill_i32(hipStream_t st, int32_t* p, size_t n, int32_t v)
     hipLaunchKernelGGL(k_fill_i32, dim3(fill_blocks(n)), dim3(kThreads), 0, st, p, n, v);
     return hipGetLastError();
 }
-
-__device__ __forceinline__ int scene_of_key(const DevParams& p, uint32_t key)
-{
-    return p.num_scenes > 1 ? (int)(key / (uint32_t)p.bps) : 0;
-}
-
-__device__ __forceinline__ void atomic_add_global(float* a, float v) { unsafeAtomicAdd(a, v); }
 
 // ===================================================================================== grid update
 
@@ -129,6 +140,8 @@ hipError_t launch_grid_update(hipStream_t st, const DevParams& p, const Grid& g)
     return hipGetLastError();
 }
 
+#endif  // MPM_MAIN_TU
+
 // ===================================================================================== per-particle helpers
 
 __device__ __forceinline__ Mat2 load_F(const Store& s, int i) { return {s.f00[i], s.f01[i], s.f10[i], s.f11[i]}; }
@@ -183,6 +196,7 @@ __device__ __forceinline__ void st_C(const Store& s, uint32_t o, const Mat2& C)
     sto(s.c11, o, C.a11);
 }
 
+#ifdef MPM_MAIN_TU
 // ===================================================================================== reference structure
 
 // P2GKernel + particleToGrid (MPM.cu:53-129): one particle per lane, 27 float atomics into HBM.
@@ -763,6 +777,8 @@ hipError_t launch_binning(hipStream_t st, const DevParams& p, const Store& src, 
                        ss.num_keys, ss.chunk, ss.items, ss.num_items);
     return hipGetLastError();
 }
+
+#endif  // MPM_MAIN_TU
 
 // ===================================================================================== binned kernels
 
@@ -1437,6 +1453,7 @@ k_g2p2g_learned(DevParams p, Store s, Grid g, const Item* items,
     flush_acc_tiles<BX, BY>(p, g, gb, ox, oy, acc);
 }
 
+#ifdef MPM_MAIN_TU
 int shape_index(int bx, int by)
 {
     for (int k = 0; k < kNumShapes; ++k) {
@@ -1445,7 +1462,9 @@ int shape_index(int bx, int by)
     return -1;
 }
 
-static unsigned item_blocks(const SortScratch& ss) { return (unsigned)(((ss.max_items + 7) / 8) * 8); }
+#endif  // MPM_MAIN_TU
+
+static inline unsigned item_blocks(const SortScratch& ss) { return (unsigned)(((ss.max_items + 7) / 8) * 8); }
 
 #define MPM_DISPATCH_SHAPE(KERNEL)                                                                              \
     switch (shape) {                                                                                            \
@@ -1460,6 +1479,7 @@ static unsigned item_blocks(const SortScratch& ss) { return (unsigned)(((ss.max_
     default: return hipErrorInvalidValue;                                                                       \
     }
 
+#ifdef MPM_MAIN_TU
 hipError_t launch_p2g_tiled(hipStream_t st, int shape, const DevParams& p, const Store& s, const Grid& g,
                             const SortScratch& ss, const int32_t* count)
 {
@@ -1486,10 +1506,19 @@ hipError_t launch_g2p2g(hipStream_t st, int shape, const DevParams& p, const Sto
         MPM_DISPATCH_SHAPE(k_g2p2g_learned)
         return hipGetLastError();
     }
+    return launch_g2p2g_plain(st, shape, p, s, g, ss, count);
+}
+#else
+hipError_t launch_g2p2g_plain(hipStream_t st, int shape, const DevParams& p, const Store& s, const Grid& g,
+                              const SortScratch& ss, const int32_t* count)
+{
+    const unsigned nb = item_blocks(ss);
     MPM_DISPATCH_SHAPE(k_g2p2g)
     return hipGetLastError();
 }
+#endif  // MPM_MAIN_TU
 
+#ifdef MPM_MAIN_TU
 // ===================================================================================== output
 
 __device__ OutView g_out_view;
@@ -1659,5 +1688,7 @@ hipError_t launch_stress(hipStream_t st, const DevParams& p, int n, const uint8_
                        J, tau, snow_return, F_out, J_out);
     return hipGetLastError();
 }
+
+#endif  // MPM_MAIN_TU
 
 }  // namespace mpm
